@@ -1,0 +1,247 @@
+"""Pin the CPU oracle against the reference's own golden vectors and KATs.
+
+Every assertion here compares the oracle (oracle/sky_oracle.c) with numbers that
+the reference's tests hold, or with what the mathematics fixes independently of
+the oracle's formula. These run on the CPU only.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from tests.fixtures import (
+    FREQS,
+    MWA_LAT_RAD,
+    NO_BEAM_CASES,
+    OFF_PHASE_CENTRE,
+    PHASE_CENTRE,
+    TYPE_MASK,
+    XYZS,
+    golden,
+    multiple_components,
+    single,
+)
+from mwa_hyperdrive_amd.srclist import (
+    CurvedPowerLaw,
+    FluxDensity,
+    FluxList,
+    Point,
+    PowerLaw,
+    Source,
+    SourceComponent,
+    SourceList,
+    radec_from_degrees,
+)
+
+
+def obs(**kw):
+    return orc.Obs(XYZS, FREQS, PHASE_CENTRE, 0.0, MWA_LAT_RAD, **kw)
+
+
+@pytest.mark.parametrize("checker,builder,pos,flux", NO_BEAM_CASES)
+def test_no_beam_goldens_bit_exact(checker, builder, pos, flux):
+    """The f32 output equals the reference's goldens with epsilon 0 (src/model/tests/cpu.rs:28)."""
+    v32, v64 = orc.model_timestep(obs(), single(builder, pos, flux), mask=TYPE_MASK[builder])
+    np.testing.assert_array_equal(v32, golden(checker))
+    np.testing.assert_array_equal(v32, v64.astype(np.float32))
+
+
+@pytest.mark.parametrize("builder,checker", [("gaussian", "test_multiple_gaussian_components"),
+                                             ("shapelet", "test_multiple_shapelet_components")])
+def test_multiple_components_goldens(builder, checker):
+    v32, _ = orc.model_timestep(obs(), multiple_components(builder), mask=TYPE_MASK[builder])
+    np.testing.assert_array_equal(v32, golden(checker))
+
+
+def test_shapelet_uvw_goldens_bit_exact():
+    """Shapelet UVWs use each component's own HA/Dec (src/model/tests/mod.rs:1530-1568, epsilon 0)."""
+    expected = np.asarray(orc_golden_uvws(), dtype=np.float64).reshape(3, 2, 3)
+    for i, (ra_deg, dec_deg) in enumerate([(1.0, -27.0), (1.1, -27.0)]):
+        ra, dec = radec_from_degrees(ra_deg, dec_deg)
+        uvw = orc.cross_uvws(XYZS, 0.0 - ra, dec)
+        np.testing.assert_array_equal(uvw[:, :2], expected[:, i, :2])
+
+
+def orc_golden_uvws():
+    from tests.fixtures import GOLDEN
+    return GOLDEN["goldens"]["shapelet_uvws_multiple"]
+
+
+def test_autos_golden():
+    """model_timestep_autos_with_point (src/model/tests/cpu.rs:440-458, mod.rs:1571-1590)."""
+    src = single("point", radec_from_degrees(1.0, -27.0), "list")
+    v32, _ = orc.model_autos(obs(), src)
+    np.testing.assert_array_equal(v32, golden("test_model_timestep_autos_with_point"))
+    for builder in ("gaussian", "shapelet"):
+        v32, _ = orc.model_autos(obs(), single(builder, radec_from_degrees(1.0, -27.0), "list"))
+        np.testing.assert_array_equal(v32, golden("test_model_timestep_autos_with_point"))
+
+
+def test_autos_sum_of_flux_densities_no_beam():
+    """src/model/tests/cpu.rs:506-576: autos equal the per-frequency flux sum, epsilon 0."""
+    def pt(pos, vals):
+        return SourceComponent(pos[0], pos[1], Point(),
+                               FluxList([FluxDensity(f, v) for f, v in zip(FREQS, vals)]))
+    sl = SourceList([("s1", Source([pt(radec_from_degrees(1.0, -27.0), [1.0, 3.0, 2.0])])),
+                     ("s2", Source([pt(radec_from_degrees(1.1, -27.0), [0.5, 0.25, 1.5])]))])
+    v32, _ = orc.model_autos(obs(), sl)
+    for fi, s in enumerate([1.5, 3.25, 3.5]):
+        for t in range(3):
+            np.testing.assert_array_equal(v32[fi, t], np.float32([s, 0, 0, 0, 0, 0, s, 0]))
+
+
+def test_update_with_a_source_semantics():
+    """update_with_a_source keeps the phase centre given; an off-centre point then gives the
+    off-zenith goldens (src/model/tests/cpu.rs:370-398)."""
+    v32, _ = orc.model_timestep(obs(), single("point", radec_from_degrees(1.0, -27.0), "list"), mask=1)
+    np.testing.assert_array_equal(v32, golden("test_list_off_zenith_visibilities"))
+
+
+def test_gaussian_envelope_pinned_by_fee_golden_ratio():
+    """With the same position and flux, (non-trivial Gaussian with FEE) / (point with FEE) cancels
+    the beam and leaves exp(K (maj^2 kx^2 + min^2 ky^2)) for each (freq, baseline)
+    (src/model/tests/mod.rs:1177-1390 over 524-597, 810-885, 1097-1171)."""
+    pairs = [("test_non_trivial_gaussian_list", "test_list_off_zenith_visibilities_fee", "list"),
+             ("test_non_trivial_gaussian_power_law", "test_power_law_off_zenith_visibilities_fee", "power_law"),
+             ("test_non_trivial_gaussian_curved_power_law", "test_curved_power_law_off_zenith_visibilities_fee",
+              "curved_power_law")]
+    for g_name, p_name, flux in pairs:
+        ratio = golden(g_name)[..., 0].astype(np.float64) / golden(p_name)[..., 0].astype(np.float64)
+        g32, g64 = orc.model_timestep(obs(), single("gaussian2", OFF_PHASE_CENTRE, flux), mask=2)
+        p32, p64 = orc.model_timestep(obs(), single("point", OFF_PHASE_CENTRE, flux), mask=1)
+        env = g64[..., 0] / p64[..., 0]
+        np.testing.assert_allclose(env, ratio, rtol=2e-6)
+
+
+def test_lmn_kat():
+    """components/tests.rs:80-82: J000042-342358 (test_files srclist) about phase centre (0, -27)."""
+    ra, dec = radec_from_degrees(0.17587049999999999, -34.40055)
+    lmn = orc.lmn_rime(ra, dec, *PHASE_CENTRE)
+    tau = 2 * math.pi
+    assert abs(lmn[0] / tau - 0.0025326811687516274) <= np.finfo(float).eps
+    assert abs(lmn[1] / tau - -0.12880688061967666) <= np.finfo(float).eps
+    assert abs(lmn[2] / tau + 1.0 - 0.9916664625927036) <= np.finfo(float).eps
+
+
+def test_flux_ratio_kats():
+    """flux_density/tests.rs:11-29."""
+    c = SourceComponent(0, 0, Point(), PowerLaw(-0.6, FluxDensity(150.0, 1.0)))
+    assert abs(orc.estimate_at_freq(c, 160.0)[1] - 0.9620170425907598) < 1e-10
+    assert abs(orc.estimate_at_freq(c, 140.0)[1] - 1.0422644718599143) < 1e-10
+
+
+def test_list_estimation_kats():
+    """flux_density/tests.rs:31-217 (negative and sign-changing list fluxes)."""
+    eps = np.finfo(float).eps
+    c = SourceComponent(0, 0, Point(), FluxList([FluxDensity(150e6, -1.0), FluxDensity(175e6, -3.0),
+                                                 FluxDensity(200e6, -2.0)]))
+    for f, want in [(140e6, -0.611583722518741), (150e6, -1.0), (160e6, -1.584007188344133),
+                    (190e6, -2.3370702845538984), (210e6, -1.72460308893938)]:
+        assert abs(orc.estimate_at_freq(c, f)[1] - want) <= eps
+    c = SourceComponent(0, 0, Point(), FluxList([FluxDensity(100e6, -1.0), FluxDensity(200e6, 1.0)]))
+    assert abs(orc.estimate_at_freq(c, 90e6)[1] - (-1.0 - 0.2)) <= eps
+    assert abs(orc.estimate_at_freq(c, 210e6)[1] - (1.0 + 0.2)) <= eps
+    assert abs(orc.estimate_at_freq(c, 150e6)[1] - 0.0) <= eps
+    c = SourceComponent(0, 0, Point(), FluxList([FluxDensity(100e6, -1.0), FluxDensity(150e6, -0.5),
+                                                 FluxDensity(200e6, 1.0)]))
+    assert abs(orc.estimate_at_freq(c, 90e6)[1] - -1.1973550404744007) <= eps
+    assert abs(orc.estimate_at_freq(c, 145e6)[1] - -0.5298337000434852) <= eps
+    assert abs(orc.estimate_at_freq(c, 155e6)[1] - (-0.5 + 1.5 / 50.0 * 5.0)) <= eps
+    assert abs(orc.estimate_at_freq(c, 210e6)[1] - (1.0 + 1.5 / 50.0 * 10.0)) <= eps
+
+
+def test_split_components_list_flux_kat():
+    """components/tests.rs:84-104: a two-entry list estimated at 180 MHz."""
+    c = SourceComponent(0, 0, Point(), FluxList([FluxDensity(80e6, 2.13017), FluxDensity(240e6, 0.33037)]))
+    si = math.log(0.33037 / 2.13017) / math.log(240e6 / 80e6)
+    want = 2.13017 * (180e6 / 80e6) ** si
+    assert abs(orc.estimate_at_freq(c, 180e6)[1] - want) <= 4 * np.finfo(float).eps
+
+
+def test_cpl_reference_frequency_kat():
+    """src/model/tests/gpu.rs:558-601: CPL with a 200 MHz reference, re-referenced to 150 MHz."""
+    c = SourceComponent(0, 0, Point(), CurvedPowerLaw(-0.8, FluxDensity(200e6, 1.0), 0.03))
+    fd150 = orc.estimate_at_freq(c, 150e6)
+    assert fd150[1] == pytest.approx(1.261912575563708, abs=np.finfo(float).eps)
+    assert orc.cpl_si_at_150mhz(c) == pytest.approx(-0.8172609243471072, abs=np.finfo(float).eps)
+
+
+def test_shapelet_table_kats_and_closed_form():
+    """src/model/shapelets.rs:44-60 samples, plus the closed form exp(-x^2/2) H_n(x)/sqrt(2^n n!)
+    evaluated independently with numpy's Hermite series for n <= 12."""
+    t = orc.shapelet_basis_table()
+    assert abs(t[0, 5000] - 1.0) < 1e-10
+    assert abs(t[0, 5001] - 0.99995) < 1e-8
+    x = np.linspace(-50, 50, 10001)
+    sel = np.abs(x) < 8
+    for n in range(13):
+        coef = np.zeros(n + 1)
+        coef[n] = 1.0
+        hn = np.polynomial.hermite.hermval(x[sel], coef)
+        want = np.exp(-0.5 * x[sel] ** 2) * hn / math.sqrt(2.0 ** n * math.factorial(n))
+        np.testing.assert_allclose(t[n, sel], want, rtol=1e-10, atol=1e-12)
+    assert np.all(t[:, :100] == 0.0) and np.all(t[:, -100:] == 0.0)
+
+
+def test_baseline_order_matches_tile_baseline_flags():
+    """src/math/mod.rs:96-109: (0,1)->0, (0,2)->1, ... over unflagged tiles."""
+    import ctypes as C
+    n = 7
+    k = 0
+    for i in range(n):
+        for j in range(i + 1, n):
+            t1, t2 = C.c_int32(), C.c_int32()
+            orc.lib().orc_baseline_to_tiles(n, k, C.byref(t1), C.byref(t2))
+            assert (t1.value, t2.value) == (i, j)
+            k += 1
+
+
+def test_unique_tiles_dedup():
+    """src/model/cpu.rs:105-143: tiles with equal (delays, gains) share a beam; delay 32 kills
+    the dipole in both polarisations (fix_amps_ndarray, cpu.rs:944-964)."""
+    delays = np.zeros((5, 16), np.uint32)
+    gains = np.ones((5, 32))
+    delays[2, 3] = 32          # dead dipole 3
+    gains[3, 3] = 0.0          # dead X dipole 3 ...
+    gains[3, 19] = 0.0         # ... and Y dipole 3 -> same amps as tile 2, but delays differ
+    gains[4, 3] = 0.0
+    gains[4, 19] = 0.0         # tile 4 equals tile 3
+    n, tm = orc.unique_tiles(5, [1], delays, gains)
+    assert n == 3
+    assert list(tm) == [0, 0, 1, 2, 2]
+
+
+def test_azel_closed_forms():
+    """HA 0, Dec = latitude is the zenith; HA 0, Dec above the latitude is due south of zenith
+    (az = 180 deg, za = Dec - lat) in the southern hemisphere."""
+    az, el = orc.azel(0.0, MWA_LAT_RAD, MWA_LAT_RAD)
+    assert abs(el - math.pi / 2) < 1e-12
+    az, el = orc.azel(0.0, MWA_LAT_RAD + 0.1, MWA_LAT_RAD)
+    assert abs((math.pi / 2 - el) - 0.1) < 1e-12
+    assert abs(az - 0.0) < 1e-12 or abs(az - 2 * math.pi) < 1e-12
+    az, el = orc.azel(0.0, MWA_LAT_RAD - 0.1, MWA_LAT_RAD)
+    assert abs(az - math.pi) < 1e-12
+
+
+def test_analytic_beam_closed_forms():
+    """Own beam model: identity at zenith for an ideal tile; a tile with every dipole dead is
+    zero; the steering delays move the peak; the array factor of 16 equal dipoles at zenith is
+    16 times one dipole's."""
+    j = orc.analytic_beam(0.0, 0.0, 180e6)
+    np.testing.assert_allclose(j, [1, 0, 0, 0, 0, 0, 1, 0], atol=1e-15)
+    dead = np.full(16, 32)
+    np.testing.assert_array_equal(orc.analytic_beam(0.3, 0.4, 180e6, dead), np.zeros(8))
+    gains = np.zeros(32)
+    gains[5] = gains[21] = 1.0
+    one = orc.analytic_beam(0.0, 0.0, 180e6, None, gains)
+    np.testing.assert_allclose(one * 16, [1, 0, 0, 0, 0, 0, 1, 0], atol=1e-14)
+    # Delays that steer 20 deg towards east: x_k sin(za) / c = d_k * 435 ps.
+    za = math.radians(20.0)
+    xs = np.array([((d % 4) - 1.5) * 1.1 for d in range(16)])
+    steer = xs * math.sin(za) / 299792458.0 / 435e-12
+    delays = np.round(steer - steer.min()).astype(np.uint32)
+    on = abs(complex(*orc.analytic_beam(math.pi / 2, za, 180e6, delays)[:2]))
+    off = abs(complex(*orc.analytic_beam(3 * math.pi / 2, za, 180e6, delays)[:2]))
+    assert on > 3 * off

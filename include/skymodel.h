@@ -1,0 +1,149 @@
+/*
+ * C ABI of the MI355X (gfx950) sky-model visibility predictor.
+ *
+ * Three layers, all returning NULL on success or a static error string (the
+ * reference's convention, src/gpu/common.cuh:552-561 and src/gpu/mod.rs:74-98):
+ *
+ *  1. Kernel-level drop-ins with the reference's exact signatures
+ *     (src/gpu/model.h:33-56). Every pointer inside the structs is device memory,
+ *     visibilities are accumulated into (+=) as JonesF32 [freq][baseline].
+ *  2. f64 variants of the same: identical inputs plus a nullable JonesF64 output
+ *     that receives the f64 accumulators before demotion (the parity path).
+ *  3. A handle-level SkyModeller (src/model/mod.rs:123-204, src/model/gpu.rs:150-1527):
+ *     construction from a source list, per-timestep modelling from (LST, latitude),
+ *     update_with_a_source, autos. Precession and Epoch->LST conversion stay with the
+ *     caller (the trait's Epoch methods call marlu for that, src/model/cpu.rs:731-769).
+ *
+ * All device work is enqueued on the HIP null stream of the calling thread's current
+ * device; entry points that take host buffers synchronise before returning.
+ */
+#pragma once
+
+#include "skymodel_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ 1. kernel drop-ins */
+
+/* Replaces `model_points` (src/gpu/model.h:324-325; launched at src/model/gpu.rs:913-945).
+ * d_beam_jones is [num_unique_tiles][num_unique_beam_freqs][directions] with directions
+ * ordered power law, curved power law, list (src/model/gpu.rs:889-910). */
+const char *model_points(const Points *comps, const Addresses *a, const UVW *d_uvws, const JonesF64 *d_beam_jones,
+                         JonesF32 *d_vis_fb);
+
+/* Replaces `model_gaussians` (src/gpu/model.h:332-333). */
+const char *model_gaussians(const Gaussians *comps, const Addresses *a, const UVW *d_uvws,
+                            const JonesF64 *d_beam_jones, JonesF32 *d_vis_fb);
+
+/* Replaces `model_shapelets` (src/gpu/model.h:340-341). The shapelet UVs inside `comps`
+ * are [baseline][component] metres, as the reference's host computes them. */
+const char *model_shapelets(const Shapelets *comps, const Addresses *a, const UVW *d_uvws,
+                            const JonesF64 *d_beam_jones, JonesF32 *d_vis_fb);
+
+/* Replaces `model_autos` (src/gpu/model.h:347). vis_fb is [freq][tile]. */
+const char *model_autos(int num_tiles, int num_freqs, const double *freqs, const Points *points,
+                        const Gaussians *gaussians, const Shapelets *shapelets, const JonesF64 *beam_jones,
+                        const int *tile_map, const int *freq_map, int num_fee_freqs,
+                        const int *tile_index_to_unflagged_tile_index_map, JonesF32 *vis_fb);
+
+/* ------------------------------------------------------------------ 2. f64 variants
+ * Same as the drop-ins; d_vis_fb_f64 (nullable, device, [freq][baseline]) is accumulated
+ * with the un-demoted f64 sums. No counterpart in the reference: it exposes the f64
+ * accumulator that SkyModellerCpu keeps before `Jones::from` (src/model/cpu.rs:335-348). */
+const char *sm_model_points_f64(const Points *comps, const Addresses *a, const UVW *d_uvws,
+                                const JonesF64 *d_beam_jones, JonesF32 *d_vis_fb, JonesF64 *d_vis_fb_f64);
+const char *sm_model_gaussians_f64(const Gaussians *comps, const Addresses *a, const UVW *d_uvws,
+                                   const JonesF64 *d_beam_jones, JonesF32 *d_vis_fb, JonesF64 *d_vis_fb_f64);
+const char *sm_model_shapelets_f64(const Shapelets *comps, const Addresses *a, const UVW *d_uvws,
+                                   const JonesF64 *d_beam_jones, JonesF32 *d_vis_fb, JonesF64 *d_vis_fb_f64);
+
+/* ------------------------------------------------------------------ 3. SkyModeller handle */
+
+enum { SM_BEAM_NONE = 0, SM_BEAM_ANALYTIC = 1 };
+/* Polarisations (src/context/mod.rs:26-33), used by mask_pols (src/model/mod.rs:264-285). */
+enum { SM_POLS_XX_XY_YX_YY = 0, SM_POLS_XX = 1, SM_POLS_YY = 2, SM_POLS_XX_YY = 3, SM_POLS_XX_YY_XY = 4 };
+
+/* Arguments of new_sky_modeller (src/model/mod.rs:215-227) minus the Epoch-only ones. */
+typedef struct SmObsParams {
+    int32_t num_unflagged_tiles;
+    const double *unflagged_tile_xyzs; /* host [num_unflagged_tiles][3], geodetic metres */
+    int32_t num_flagged_tiles;
+    const int32_t *flagged_tiles;      /* host, total-array tile indices */
+    int32_t num_freqs;
+    const double *freqs;               /* host, unflagged fine-channel frequencies, Hz */
+    double phase_centre_ra, phase_centre_dec;
+    double array_longitude, array_latitude;
+    int32_t beam_type;                 /* SM_BEAM_* */
+    const uint32_t *dipole_delays;     /* host [total tiles][16] or NULL (all 0) */
+    const double *dipole_gains;        /* host [total tiles][32] or NULL (all 1) */
+    int32_t pols;                      /* SM_POLS_* */
+} SmObsParams;
+
+typedef struct SmModeller SmModeller;
+
+/* Replaces new_sky_modeller / SkyModellerGpu::new (src/model/mod.rs:215-261,
+ * src/model/gpu.rs:150-519). comps are in source-list order; source_offsets has
+ * num_sources + 1 entries. The source order is reversed internally, as the reference does
+ * (src/model/gpu.rs:230-241). */
+const char *sm_new(const SmObsParams *params, const SmComponent *comps, const int32_t *source_offsets,
+                   int32_t num_sources, SmModeller **out);
+
+/* Frees the modeller and its device memory. */
+void sm_free(SmModeller *m);
+
+/* Replaces SkyModeller::update_with_a_source (src/model/mod.rs:185-193,
+ * src/model/gpu.rs:1498-1504): the sky becomes this one source's components, modelled
+ * about the new phase centre. */
+const char *sm_update_with_a_source(SmModeller *m, const SmComponent *comps, int32_t num_comps, double pc_ra,
+                                    double pc_dec);
+
+/* Replaces SkyModeller::model_timestep_with (src/model/mod.rs:150-163) for a given LST and
+ * latitude (no precession). vis_fb: host [num_freqs][num_baselines] JonesF32, accumulated
+ * into; vis_fb_f64 (nullable, host) receives the f64 sums. uvws_out (nullable, host)
+ * receives the baseline UVWs in metres. mask_pols is applied to vis_fb afterwards. */
+const char *sm_model_timestep_with(SmModeller *m, double lst, double latitude, UVW *uvws_out, JonesF32 *vis_fb,
+                                   JonesF64 *vis_fb_f64);
+
+/* Device-buffer form used by device-resident callers (src/model/gpu.rs:1152-1166,
+ * src/params/peel/gpu.rs:447-459): d_vis_fb / d_vis_fb_f64 are device memory; the call
+ * returns once the work is enqueued. d_uvws (nullable) overrides the UVWs computed from the
+ * modeller's tile positions. No mask_pols, like the reference's device method. */
+const char *sm_model_timestep_with_device(SmModeller *m, double lst, double latitude, const UVW *d_uvws,
+                                          JonesF32 *d_vis_fb, JonesF64 *d_vis_fb_f64);
+
+/* Replaces SkyModeller::model_timestep_autos_with (src/model/mod.rs:195-203). vis_fb: host
+ * [num_freqs][num_unflagged_tiles]. */
+const char *sm_model_timestep_autos_with(SmModeller *m, double lst, double latitude, JonesF32 *vis_fb,
+                                         JonesF64 *vis_fb_f64);
+
+/* Sizes and the accumulated device time of the modelling kernels (HIP events on the
+ * launch stream). No counterpart in the reference. */
+typedef struct SmInfo {
+    int32_t num_baselines;
+    int32_t num_freqs;
+    int32_t num_unflagged_tiles;
+    int32_t num_unique_tiles;
+    int32_t num_points, num_gaussians, num_shapelets;
+    int32_t num_kernel_launches; /* direct modelling kernels timed since the last reset */
+    double kernel_ms;            /* their summed duration */
+    double kernel_ms_points, kernel_ms_gaussians, kernel_ms_shapelets;
+} SmInfo;
+const char *sm_get_info(SmModeller *m, SmInfo *info);
+/* Enables (1) or disables (0) per-kernel HIP-event timing and resets the counters. */
+const char *sm_set_timing(SmModeller *m, int32_t enabled);
+const char *sm_synchronize(void);
+
+/* ------------------------------------------------------------------ beam */
+/* Analytic MWA-like beam on the device: out[tile][freq][dir] JonesF64 for the given unique
+ * tiles (delays [n_tiles][16], gains [n_tiles][32], host), frequencies (host, Hz) and
+ * directions (device az/za, radians). Identity at zenith for an ideal tile. Stands in for
+ * BeamGpu::calc_jones_pair (src/beam/mod.rs:143-177). */
+const char *sm_analytic_beam(int32_t n_tiles, const uint32_t *delays, const double *gains, int32_t n_freqs,
+                             const double *freqs, int32_t n_dirs, const double *d_az, const double *d_za,
+                             JonesF64 *d_out);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif

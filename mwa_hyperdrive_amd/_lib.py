@@ -1,0 +1,181 @@
+"""Loader and ctypes signatures of the HIP library (include/skymodel.h).
+
+There is no CPU fallback: if libskymodel_hip.so is missing or cannot be loaded
+this raises, so nothing silently runs elsewhere.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+from .srclist import SmComponent, ShapeletCoeff
+
+LIB_PATH = Path(__file__).resolve().parent / "libskymodel_hip.so"
+_lib = None
+
+
+class SkyModelError(RuntimeError):
+    """A C-ABI entry point returned an error string (the reference's GpuError::Kernel)."""
+
+
+class UVW(C.Structure):
+    _fields_ = [("u", C.c_double), ("v", C.c_double), ("w", C.c_double)]
+
+
+class LmnRime(C.Structure):
+    _fields_ = [("l", C.c_double), ("m", C.c_double), ("n", C.c_double)]
+
+
+class GaussianParams(C.Structure):
+    _fields_ = [("maj", C.c_double), ("min", C.c_double), ("pa", C.c_double)]
+
+
+class Addresses(C.Structure):
+    _fields_ = [
+        ("num_freqs", C.c_int),
+        ("num_vis", C.c_int),
+        ("num_baselines", C.c_int),
+        ("d_freqs", C.c_void_p),
+        ("d_shapelet_basis_values", C.c_void_p),
+        ("num_unique_beam_freqs", C.c_int),
+        ("d_tile_map", C.c_void_p),
+        ("d_freq_map", C.c_void_p),
+        ("d_tile_index_to_unflagged_tile_index_map", C.c_void_p),
+    ]
+
+
+class Points(C.Structure):
+    _fields_ = [
+        ("num_power_laws", C.c_int), ("power_law_lmns", C.c_void_p), ("power_law_fds", C.c_void_p),
+        ("power_law_sis", C.c_void_p),
+        ("num_curved_power_laws", C.c_int), ("curved_power_law_lmns", C.c_void_p),
+        ("curved_power_law_fds", C.c_void_p), ("curved_power_law_sis", C.c_void_p),
+        ("curved_power_law_qs", C.c_void_p),
+        ("num_lists", C.c_int), ("list_lmns", C.c_void_p), ("list_fds", C.c_void_p),
+    ]
+
+
+class Gaussians(C.Structure):
+    _fields_ = [
+        ("num_power_laws", C.c_int), ("power_law_lmns", C.c_void_p), ("power_law_fds", C.c_void_p),
+        ("power_law_sis", C.c_void_p), ("power_law_gps", C.c_void_p),
+        ("num_curved_power_laws", C.c_int), ("curved_power_law_lmns", C.c_void_p),
+        ("curved_power_law_fds", C.c_void_p), ("curved_power_law_sis", C.c_void_p),
+        ("curved_power_law_qs", C.c_void_p), ("curved_power_law_gps", C.c_void_p),
+        ("num_lists", C.c_int), ("list_lmns", C.c_void_p), ("list_fds", C.c_void_p), ("list_gps", C.c_void_p),
+    ]
+
+
+class Shapelets(C.Structure):
+    _fields_ = [
+        ("num_power_laws", C.c_int), ("power_law_lmns", C.c_void_p), ("power_law_fds", C.c_void_p),
+        ("power_law_sis", C.c_void_p), ("power_law_gps", C.c_void_p), ("power_law_shapelet_uvs", C.c_void_p),
+        ("power_law_shapelet_coeffs", C.c_void_p), ("power_law_num_shapelet_coeffs", C.c_void_p),
+        ("num_curved_power_laws", C.c_int), ("curved_power_law_lmns", C.c_void_p),
+        ("curved_power_law_fds", C.c_void_p), ("curved_power_law_sis", C.c_void_p),
+        ("curved_power_law_qs", C.c_void_p), ("curved_power_law_gps", C.c_void_p),
+        ("curved_power_law_shapelet_uvs", C.c_void_p), ("curved_power_law_shapelet_coeffs", C.c_void_p),
+        ("curved_power_law_num_shapelet_coeffs", C.c_void_p),
+        ("num_lists", C.c_int), ("list_lmns", C.c_void_p), ("list_fds", C.c_void_p), ("list_gps", C.c_void_p),
+        ("list_shapelet_uvs", C.c_void_p), ("list_shapelet_coeffs", C.c_void_p),
+        ("list_num_shapelet_coeffs", C.c_void_p),
+    ]
+
+
+class SmObsParams(C.Structure):
+    _fields_ = [
+        ("num_unflagged_tiles", C.c_int32),
+        ("unflagged_tile_xyzs", C.c_void_p),
+        ("num_flagged_tiles", C.c_int32),
+        ("flagged_tiles", C.c_void_p),
+        ("num_freqs", C.c_int32),
+        ("freqs", C.c_void_p),
+        ("phase_centre_ra", C.c_double),
+        ("phase_centre_dec", C.c_double),
+        ("array_longitude", C.c_double),
+        ("array_latitude", C.c_double),
+        ("beam_type", C.c_int32),
+        ("dipole_delays", C.c_void_p),
+        ("dipole_gains", C.c_void_p),
+        ("pols", C.c_int32),
+    ]
+
+
+class SmInfo(C.Structure):
+    _fields_ = [
+        ("num_baselines", C.c_int32),
+        ("num_freqs", C.c_int32),
+        ("num_unflagged_tiles", C.c_int32),
+        ("num_unique_tiles", C.c_int32),
+        ("num_points", C.c_int32),
+        ("num_gaussians", C.c_int32),
+        ("num_shapelets", C.c_int32),
+        ("num_kernel_launches", C.c_int32),
+        ("kernel_ms", C.c_double),
+        ("kernel_ms_points", C.c_double),
+        ("kernel_ms_gaussians", C.c_double),
+        ("kernel_ms_shapelets", C.c_double),
+    ]
+
+
+# Every symbol the library exports through include/*.h, with its ctypes signature.
+_P = C.c_void_p
+_D = C.c_double
+_I = C.c_int32
+SIGNATURES = {
+    "model_points": [_P, _P, _P, _P, _P],
+    "model_gaussians": [_P, _P, _P, _P, _P],
+    "model_shapelets": [_P, _P, _P, _P, _P],
+    "model_autos": [C.c_int, C.c_int, _P, _P, _P, _P, _P, _P, _P, C.c_int, _P, _P],
+    "sm_model_points_f64": [_P, _P, _P, _P, _P, _P],
+    "sm_model_gaussians_f64": [_P, _P, _P, _P, _P, _P],
+    "sm_model_shapelets_f64": [_P, _P, _P, _P, _P, _P],
+    "sm_new": [C.POINTER(SmObsParams), _P, _P, _I, C.POINTER(_P)],
+    "sm_update_with_a_source": [_P, _P, _I, _D, _D],
+    "sm_model_timestep_with": [_P, _D, _D, _P, _P, _P],
+    "sm_model_timestep_with_device": [_P, _D, _D, _P, _P, _P],
+    "sm_model_timestep_autos_with": [_P, _D, _D, _P, _P],
+    "sm_get_info": [_P, C.POINTER(SmInfo)],
+    "sm_set_timing": [_P, _I],
+    "sm_synchronize": [],
+    "sm_analytic_beam": [_I, _P, _P, _I, _P, _I, _P, _P, _P],
+    "sm_model_points_factored": [_P, _D, _D, _I, _P, _P],
+    "sm_factored_info": [_P, _P],
+    "sm_surrogate_forward": [_P, _I, _P, _P, _I],
+    "sm_surrogate_train_step": [_P, _P, _P, _I, _P],
+    "sm_surrogate_new": [_I, _I, _I, _I, C.POINTER(_P)],
+    "sm_surrogate_free": [_P],
+    "sm_surrogate_params": [_P, _P, _P, _P],
+    "sm_surrogate_set_params": [_P, _P],
+    "sm_beam_features": [_I, _P, _P, _D, _P, _P, _P, _I],
+}
+VOID_RETURN = {"sm_free", "sm_surrogate_free"}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise SkyModelError(
+                f"{LIB_PATH} is missing: build it with `python -m mwa_hyperdrive_amd.build` "
+                "(there is no CPU fallback)")
+        L = C.CDLL(str(LIB_PATH))
+        for name, args in SIGNATURES.items():
+            if hasattr(L, name):
+                fn = getattr(L, name)
+                fn.argtypes = args
+                fn.restype = C.c_char_p
+        L.sm_free.argtypes = [_P]
+        L.sm_free.restype = None
+        _lib = L
+    return _lib
+
+
+def check(err) -> None:
+    if err is not None:
+        raise SkyModelError(err.decode() if isinstance(err, bytes) else str(err))
+
+
+def call(name, *args) -> None:
+    check(getattr(lib(), name)(*args))

@@ -1,0 +1,434 @@
+// Kernel-level drop-ins with the reference's signatures (src/gpu/model.h:33-56) and the
+// shared per-component-type driver used by both these and the SkyModeller handle.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "sm_abi_common.h"
+
+namespace sm {
+
+hipError_t DevBuf::reserve(size_t n) {
+    if (n <= bytes && ptr != nullptr) return hipSuccess;
+    release();
+    size_t want = n < 256 ? 256 : n;
+    hipError_t e = hipMalloc(&ptr, want);
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        bytes = 0;
+        return e;
+    }
+    bytes = want;
+    return hipSuccess;
+}
+
+void DevBuf::release() {
+    if (ptr != nullptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+int pad_channels(int nchan) { return ((nchan + SM_NCH - 1) / SM_NCH) * SM_NCH + SM_NCH; }
+
+void pad_freqs(const double *freqs, int nchan, double *out) {
+    const int npad = pad_channels(nchan);
+    for (int f = 0; f < nchan; ++f) out[f] = freqs[f];
+    const double step = nchan >= 2 ? freqs[nchan - 1] - freqs[nchan - 2] : 1.0;
+    for (int f = nchan; f < npad; ++f) out[f] = freqs[nchan - 1] + (double)(f - nchan + 1) * step;
+}
+
+// ------------------------------------------------------------------ parameter gathers
+__global__ void k_gather_lmn(int n_pl, const LmnRime *pl, int n_cpl, const LmnRime *cpl, int n_list,
+                             const LmnRime *list, double4 *out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = n_pl + n_cpl + n_list;
+    if (c >= n) return;
+    const LmnRime l = c < n_pl ? pl[c] : (c < n_pl + n_cpl ? cpl[c - n_pl] : list[c - n_pl - n_cpl]);
+    out[c] = make_double4(l.l, l.m, l.n, 0.0);
+}
+
+// Gaussian envelope exponent coefficients (src/model/cpu.rs:456-465 expanded as a quadratic
+// form in the baseline's (u, v) metres, with the 1/c^2 of the wavelength scaling folded in).
+__device__ __host__ inline double4 gaussian_shape(GaussianParams g) {
+    const double K = -(1.5707963267948966 * 1.5707963267948966) / 0.6931471805599453;
+    const double s = sin(g.pa), c = cos(g.pa);
+    const double maj2 = g.maj * g.maj, min2 = g.min * g.min;
+    const double ic2 = 1.0 / (SM_VEL_C * SM_VEL_C);
+    return make_double4(K * (maj2 * s * s + min2 * c * c) * ic2, K * (2.0 * s * c * (maj2 - min2)) * ic2,
+                        K * (maj2 * c * c + min2 * s * s) * ic2, 0.0);
+}
+
+// Shapelet position scales (src/model/cpu.rs:638-644): (sin pa, cos pa, maj K / c, -min K / c).
+__device__ __host__ inline double4 shapelet_shape(GaussianParams g) {
+    const double K = 2.6682231283184983 / SM_SBF_DX;
+    return make_double4(sin(g.pa), cos(g.pa), g.maj * K / SM_VEL_C, -g.min * K / SM_VEL_C);
+}
+
+__global__ void k_gather_shape(int kind, int n_pl, const GaussianParams *pl, int n_cpl, const GaussianParams *cpl,
+                               int n_list, const GaussianParams *list, double4 *out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = n_pl + n_cpl + n_list;
+    if (c >= n) return;
+    const GaussianParams g = c < n_pl ? pl[c] : (c < n_pl + n_cpl ? cpl[c - n_pl] : list[c - n_pl - n_cpl]);
+    out[c] = kind == KIND_GAUSSIAN ? gaussian_shape(g) : shapelet_shape(g);
+}
+
+double4 host_gaussian_shape(GaussianParams g) { return gaussian_shape(g); }
+double4 host_shapelet_shape(GaussianParams g) { return shapelet_shape(g); }
+
+// ------------------------------------------------------------------ shared driver
+hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf &coh, JonesF32 *vis32,
+                    JonesF64 *vis64, hipStream_t stream, hipEvent_t ev0, hipEvent_t ev1) {
+    const int n = in.fb.n_pl + in.fb.n_cpl + in.fb.n_list;
+    if (n == 0 || ctx.nbl == 0 || ctx.nchan == 0) return hipSuccess;
+    hipError_t e = coh.reserve((size_t)n * ctx.nchan_pad * 8 * sizeof(double));
+    if (e != hipSuccess) return e;
+    const bool per_tile = ctx.beam.jones != nullptr && ctx.beam.uniform_row < 0;
+    e = launch_coherency(n, ctx.nchan, ctx.nchan_pad, ctx.d_freqs_pad, in.fb, ctx.beam, !per_tile, coh.as<double>(),
+                         stream);
+    if (e != hipSuccess) return e;
+    DirectArgs a;
+    a.nbl = ctx.nbl;
+    a.nchan = ctx.nchan;
+    a.nchan_pad = ctx.nchan_pad;
+    a.ncomp = n;
+    a.freqs = ctx.d_freqs_pad;
+    a.uvw = ctx.d_uvw;
+    a.lmn = in.lmn;
+    a.shape = in.shape;
+    a.coh = coh.as<double>();
+    a.bl_uv = in.bl_uv;
+    a.coeff_range = in.coeff_range;
+    a.coeffs = in.coeffs;
+    a.sbf = ctx.d_sbf;
+    a.beam = ctx.beam;
+    a.vis32 = vis32;
+    a.vis64 = vis64;
+    if (ev0 != nullptr) (void)hipEventRecord(ev0, stream);
+    e = launch_direct(kind, per_tile, ctx.even, a, stream);
+    if (ev1 != nullptr) (void)hipEventRecord(ev1, stream);
+    return e;
+}
+
+// ------------------------------------------------------------------ ABI context
+namespace {
+
+struct AbiState {
+    DevBuf freqs_pad, lmn, shape, coh, bl_rows, bl_uv, coeffs, coeff_range, sbf, accum, tile_rows;
+    std::vector<double> h_freqs_pad;
+};
+AbiState &abi_state() {
+    static AbiState s;
+    return s;
+}
+
+// Builds the per-call context from an Addresses struct: frequencies (padded to whole
+// chunks), and whether every unflagged tile uses the same beam row.
+const char *abi_ctx(const Addresses *a, const UVW *d_uvws, const JonesF64 *d_beam, int n_dir, ModelCtx &ctx) {
+    AbiState &st = abi_state();
+    ctx = ModelCtx{};
+    ctx.nbl = a->num_baselines;
+    ctx.nchan = a->num_freqs;
+    if (ctx.nbl <= 0 || ctx.nchan <= 0) return nullptr;
+    const int ntiles = (int)llround((1.0 + std::sqrt(1.0 + 8.0 * (double)ctx.nbl)) / 2.0);
+    if (ntiles * (ntiles - 1) / 2 != ctx.nbl) return "num_baselines is not a triangular number";
+    ctx.nchan_pad = pad_channels(ctx.nchan);
+    std::vector<double> f(ctx.nchan);
+    SM_TRY(hipMemcpy(f.data(), a->d_freqs, sizeof(double) * ctx.nchan, hipMemcpyDeviceToHost));
+    st.h_freqs_pad.assign(ctx.nchan_pad, 0.0);
+    pad_freqs(f.data(), ctx.nchan, st.h_freqs_pad.data());
+    SM_TRY(st.freqs_pad.reserve(sizeof(double) * ctx.nchan_pad));
+    SM_TRY(hipMemcpy(st.freqs_pad.ptr, st.h_freqs_pad.data(), sizeof(double) * ctx.nchan_pad, hipMemcpyHostToDevice));
+    ctx.d_freqs_pad = st.freqs_pad.as<double>();
+    ctx.even = chunks_even(st.h_freqs_pad.data(), ctx.nchan);
+    ctx.d_uvw = d_uvws;
+    ctx.d_sbf = a->d_shapelet_basis_values;
+    ctx.ntiles = ntiles;
+    BeamDesc &b = ctx.beam;
+    b.jones = d_beam;
+    b.freq_map = a->d_freq_map;
+    b.n_ufreq = a->num_unique_beam_freqs;
+    b.n_dir = n_dir;
+    b.dir_offset = 0;
+    b.uniform_row = 0;
+    if (d_beam == nullptr) return nullptr;
+    std::vector<int> unflagged_to_total(ntiles);
+    SM_TRY(hipMemcpy(unflagged_to_total.data(), a->d_tile_index_to_unflagged_tile_index_map, sizeof(int) * ntiles,
+                     hipMemcpyDeviceToHost));
+    int max_total = 0;
+    for (int t : unflagged_to_total) {
+        if (t < 0) return "negative tile index in tile_index_to_unflagged_tile_index_map";
+        max_total = t > max_total ? t : max_total;
+    }
+    std::vector<int> tile_map(max_total + 1);
+    SM_TRY(hipMemcpy(tile_map.data(), a->d_tile_map, sizeof(int) * (max_total + 1), hipMemcpyDeviceToHost));
+    std::vector<int> rows(ntiles);
+    bool uniform = true;
+    for (int t = 0; t < ntiles; ++t) {
+        rows[t] = tile_map[unflagged_to_total[t]];
+        if (rows[t] != rows[0]) uniform = false;
+    }
+    SM_TRY(st.tile_rows.reserve(sizeof(int) * ntiles));
+    SM_TRY(hipMemcpy(st.tile_rows.ptr, rows.data(), sizeof(int) * ntiles, hipMemcpyHostToDevice));
+    b.tile_rows = st.tile_rows.as<int>();
+    if (uniform) {
+        b.uniform_row = rows[0];
+        return nullptr;
+    }
+    b.uniform_row = -1;
+    std::vector<int2> bl_rows(ctx.nbl);
+    int k = 0;
+    for (int i = 0; i < ntiles; ++i)
+        for (int j = i + 1; j < ntiles; ++j) bl_rows[k++] = make_int2(rows[i], rows[j]);
+    SM_TRY(st.bl_rows.reserve(sizeof(int2) * ctx.nbl));
+    SM_TRY(hipMemcpy(st.bl_rows.ptr, bl_rows.data(), sizeof(int2) * ctx.nbl, hipMemcpyHostToDevice));
+    b.bl_rows = st.bl_rows.as<int2>();
+    return nullptr;
+}
+
+const char *gather_lmn(int n_pl, const LmnRime *pl, int n_cpl, const LmnRime *cpl, int n_list, const LmnRime *list,
+                       DevBuf &out) {
+    const int n = n_pl + n_cpl + n_list;
+    SM_TRY(out.reserve(sizeof(double4) * n));
+    hipLaunchKernelGGL(k_gather_lmn, dim3((n + 255) / 256), dim3(256), 0, 0, n_pl, pl, n_cpl, cpl, n_list, list,
+                       out.as<double4>());
+    SM_TRY(hipGetLastError());
+    return nullptr;
+}
+
+const char *gather_shape(Kind kind, int n_pl, const GaussianParams *pl, int n_cpl, const GaussianParams *cpl,
+                         int n_list, const GaussianParams *list, DevBuf &out) {
+    const int n = n_pl + n_cpl + n_list;
+    SM_TRY(out.reserve(sizeof(double4) * n));
+    hipLaunchKernelGGL(k_gather_shape, dim3((n + 255) / 256), dim3(256), 0, 0, (int)kind, n_pl, pl, n_cpl, cpl,
+                       n_list, list, out.as<double4>());
+    SM_TRY(hipGetLastError());
+    return nullptr;
+}
+
+}  // namespace
+}  // namespace sm
+
+using namespace sm;
+
+extern "C" const char *sm_model_points_f64(const Points *p, const Addresses *a, const UVW *d_uvws,
+                                           const JonesF64 *d_beam_jones, JonesF32 *d_vis_fb, JonesF64 *d_vis_fb_f64) {
+    if (p == nullptr || a == nullptr) return "null argument";
+    const int n = p->num_power_laws + p->num_curved_power_laws + p->num_lists;
+    if (n == 0) return nullptr;
+    ModelCtx ctx;
+    if (const char *err = abi_ctx(a, d_uvws, d_beam_jones, n, ctx)) return err;
+    if (ctx.nbl == 0 || ctx.nchan == 0) return nullptr;
+    AbiState &st = abi_state();
+    if (const char *err = gather_lmn(p->num_power_laws, p->power_law_lmns, p->num_curved_power_laws,
+                                     p->curved_power_law_lmns, p->num_lists, p->list_lmns, st.lmn))
+        return err;
+    TypeInputs in{};
+    in.fb = FluxBuckets{p->num_power_laws, p->num_curved_power_laws, p->num_lists, p->power_law_fds,
+                        p->power_law_sis, p->curved_power_law_fds, p->curved_power_law_sis, p->curved_power_law_qs,
+                        p->list_fds};
+    in.lmn = st.lmn.as<double4>();
+    SM_TRY(run_type(KIND_POINT, in, ctx, st.coh, d_vis_fb, d_vis_fb_f64, 0, nullptr, nullptr));
+    SM_TRY(hipStreamSynchronize(0));
+    return nullptr;
+}
+
+extern "C" const char *model_points(const Points *comps, const Addresses *a, const UVW *d_uvws,
+                                    const JonesF64 *d_beam_jones, JonesF32 *d_vis_fb) {
+    return sm_model_points_f64(comps, a, d_uvws, d_beam_jones, d_vis_fb, nullptr);
+}
+
+extern "C" const char *sm_model_gaussians_f64(const Gaussians *g, const Addresses *a, const UVW *d_uvws,
+                                              const JonesF64 *d_beam_jones, JonesF32 *d_vis_fb,
+                                              JonesF64 *d_vis_fb_f64) {
+    if (g == nullptr || a == nullptr) return "null argument";
+    const int n = g->num_power_laws + g->num_curved_power_laws + g->num_lists;
+    if (n == 0) return nullptr;
+    ModelCtx ctx;
+    if (const char *err = abi_ctx(a, d_uvws, d_beam_jones, n, ctx)) return err;
+    if (ctx.nbl == 0 || ctx.nchan == 0) return nullptr;
+    AbiState &st = abi_state();
+    if (const char *err = gather_lmn(g->num_power_laws, g->power_law_lmns, g->num_curved_power_laws,
+                                     g->curved_power_law_lmns, g->num_lists, g->list_lmns, st.lmn))
+        return err;
+    if (const char *err = gather_shape(KIND_GAUSSIAN, g->num_power_laws, g->power_law_gps, g->num_curved_power_laws,
+                                       g->curved_power_law_gps, g->num_lists, g->list_gps, st.shape))
+        return err;
+    TypeInputs in{};
+    in.fb = FluxBuckets{g->num_power_laws, g->num_curved_power_laws, g->num_lists, g->power_law_fds,
+                        g->power_law_sis, g->curved_power_law_fds, g->curved_power_law_sis, g->curved_power_law_qs,
+                        g->list_fds};
+    in.lmn = st.lmn.as<double4>();
+    in.shape = st.shape.as<double4>();
+    SM_TRY(run_type(KIND_GAUSSIAN, in, ctx, st.coh, d_vis_fb, d_vis_fb_f64, 0, nullptr, nullptr));
+    SM_TRY(hipStreamSynchronize(0));
+    return nullptr;
+}
+
+extern "C" const char *model_gaussians(const Gaussians *comps, const Addresses *a, const UVW *d_uvws,
+                                       const JonesF64 *d_beam_jones, JonesF32 *d_vis_fb) {
+    return sm_model_gaussians_f64(comps, a, d_uvws, d_beam_jones, d_vis_fb, nullptr);
+}
+
+extern "C" const char *sm_model_shapelets_f64(const Shapelets *s, const Addresses *a, const UVW *d_uvws,
+                                              const JonesF64 *d_beam_jones, JonesF32 *d_vis_fb,
+                                              JonesF64 *d_vis_fb_f64) {
+    if (s == nullptr || a == nullptr) return "null argument";
+    const int n_pl = s->num_power_laws, n_cpl = s->num_curved_power_laws, n_list = s->num_lists;
+    const int n = n_pl + n_cpl + n_list;
+    if (n == 0) return nullptr;
+    if (a->d_shapelet_basis_values == nullptr) return "d_shapelet_basis_values is null";
+    ModelCtx ctx;
+    if (const char *err = abi_ctx(a, d_uvws, d_beam_jones, n, ctx)) return err;
+    if (ctx.nbl == 0 || ctx.nchan == 0) return nullptr;
+    AbiState &st = abi_state();
+    if (const char *err = gather_lmn(n_pl, s->power_law_lmns, n_cpl, s->curved_power_law_lmns, n_list, s->list_lmns,
+                                     st.lmn))
+        return err;
+    if (const char *err = gather_shape(KIND_SHAPELET, n_pl, s->power_law_gps, n_cpl, s->curved_power_law_gps, n_list,
+                                       s->list_gps, st.shape))
+        return err;
+    SM_TRY(st.bl_uv.reserve(sizeof(double2) * (size_t)n * ctx.nbl));
+    SM_TRY(launch_repack_shapelet_uv(ctx.nbl, n_pl, n_cpl, n_list, s->power_law_shapelet_uvs,
+                                     s->curved_power_law_shapelet_uvs, s->list_shapelet_uvs, st.bl_uv.as<double2>(),
+                                     0));
+    // Coefficient counts per bucket -> one flattened list with (offset, count) per component.
+    const int *counts_d[3] = {s->power_law_num_shapelet_coeffs, s->curved_power_law_num_shapelet_coeffs,
+                              s->list_num_shapelet_coeffs};
+    const ShapeletCoeff *coeffs_d[3] = {s->power_law_shapelet_coeffs, s->curved_power_law_shapelet_coeffs,
+                                        s->list_shapelet_coeffs};
+    const int nb[3] = {n_pl, n_cpl, n_list};
+    std::vector<int2> ranges(n);
+    std::vector<size_t> bucket_total(3, 0);
+    int c = 0;
+    size_t off = 0;
+    for (int b = 0; b < 3; ++b) {
+        if (nb[b] == 0) continue;
+        std::vector<int> cnt(nb[b]);
+        SM_TRY(hipMemcpy(cnt.data(), counts_d[b], sizeof(int) * nb[b], hipMemcpyDeviceToHost));
+        for (int i = 0; i < nb[b]; ++i) {
+            if (cnt[i] < 0) return "negative shapelet coefficient count";
+            ranges[c++] = make_int2((int)off, cnt[i]);
+            off += (size_t)cnt[i];
+            bucket_total[b] += (size_t)cnt[i];
+        }
+    }
+    SM_TRY(st.coeffs.reserve(sizeof(ShapeletCoeff) * (off > 0 ? off : 1)));
+    size_t dst = 0;
+    for (int b = 0; b < 3; ++b) {
+        if (bucket_total[b] == 0) continue;
+        SM_TRY(hipMemcpy(st.coeffs.as<ShapeletCoeff>() + dst, coeffs_d[b], sizeof(ShapeletCoeff) * bucket_total[b],
+                         hipMemcpyDeviceToDevice));
+        dst += bucket_total[b];
+    }
+    SM_TRY(st.coeff_range.reserve(sizeof(int2) * n));
+    SM_TRY(hipMemcpy(st.coeff_range.ptr, ranges.data(), sizeof(int2) * n, hipMemcpyHostToDevice));
+    TypeInputs in{};
+    in.fb = FluxBuckets{n_pl, n_cpl, n_list, s->power_law_fds, s->power_law_sis, s->curved_power_law_fds,
+                        s->curved_power_law_sis, s->curved_power_law_qs, s->list_fds};
+    in.lmn = st.lmn.as<double4>();
+    in.shape = st.shape.as<double4>();
+    in.bl_uv = st.bl_uv.as<double2>();
+    in.coeff_range = st.coeff_range.as<int2>();
+    in.coeffs = st.coeffs.as<ShapeletCoeff>();
+    SM_TRY(run_type(KIND_SHAPELET, in, ctx, st.coh, d_vis_fb, d_vis_fb_f64, 0, nullptr, nullptr));
+    SM_TRY(hipStreamSynchronize(0));
+    return nullptr;
+}
+
+extern "C" const char *model_shapelets(const Shapelets *comps, const Addresses *a, const UVW *d_uvws,
+                                       const JonesF64 *d_beam_jones, JonesF32 *d_vis_fb) {
+    return sm_model_shapelets_f64(comps, a, d_uvws, d_beam_jones, d_vis_fb, nullptr);
+}
+
+// Autos over all three types with one demotion of the summed f64 accumulator, as the
+// reference's GPU kernel does (src/gpu/model.cu:368-485). beam_jones holds every direction
+// in the order points, Gaussians, shapelets (src/model/gpu.rs:1184-1226).
+extern "C" const char *model_autos(int num_tiles, int num_freqs, const double *freqs, const Points *points,
+                                   const Gaussians *gaussians, const Shapelets *shapelets,
+                                   const JonesF64 *beam_jones, const int *tile_map, const int *freq_map,
+                                   int num_fee_freqs, const int *tile_index_to_unflagged_tile_index_map,
+                                   JonesF32 *vis_fb) {
+    if (points == nullptr || gaussians == nullptr || shapelets == nullptr) return "null argument";
+    if (num_tiles <= 0 || num_freqs <= 0) return nullptr;
+    AbiState &st = abi_state();
+    const int np = points->num_power_laws + points->num_curved_power_laws + points->num_lists;
+    const int ng = gaussians->num_power_laws + gaussians->num_curved_power_laws + gaussians->num_lists;
+    const int ns = shapelets->num_power_laws + shapelets->num_curved_power_laws + shapelets->num_lists;
+    const int n_dir = np + ng + ns;
+    if (n_dir == 0) return nullptr;
+    const int nchan_pad = pad_channels(num_freqs);
+    std::vector<double> f(num_freqs);
+    SM_TRY(hipMemcpy(f.data(), freqs, sizeof(double) * num_freqs, hipMemcpyDeviceToHost));
+    std::vector<double> fp(nchan_pad);
+    pad_freqs(f.data(), num_freqs, fp.data());
+    SM_TRY(st.freqs_pad.reserve(sizeof(double) * nchan_pad));
+    SM_TRY(hipMemcpy(st.freqs_pad.ptr, fp.data(), sizeof(double) * nchan_pad, hipMemcpyHostToDevice));
+    std::vector<int> unflagged_to_total(num_tiles);
+    SM_TRY(hipMemcpy(unflagged_to_total.data(), tile_index_to_unflagged_tile_index_map, sizeof(int) * num_tiles,
+                     hipMemcpyDeviceToHost));
+    int max_total = 0;
+    for (int t : unflagged_to_total) max_total = t > max_total ? t : max_total;
+    std::vector<int> tmap(max_total + 1);
+    SM_TRY(hipMemcpy(tmap.data(), tile_map, sizeof(int) * (max_total + 1), hipMemcpyDeviceToHost));
+    std::vector<int> rows(num_tiles);
+    for (int t = 0; t < num_tiles; ++t) rows[t] = tmap[unflagged_to_total[t]];
+    SM_TRY(st.tile_rows.reserve(sizeof(int) * num_tiles));
+    SM_TRY(hipMemcpy(st.tile_rows.ptr, rows.data(), sizeof(int) * num_tiles, hipMemcpyHostToDevice));
+    const size_t nvis = (size_t)num_tiles * num_freqs;
+    SM_TRY(st.accum.reserve(sizeof(double) * 8 * nvis));
+    SM_TRY(hipMemsetAsync(st.accum.ptr, 0, sizeof(double) * 8 * nvis, 0));
+    BeamDesc beam{};
+    beam.jones = beam_jones;
+    beam.freq_map = freq_map;
+    beam.n_ufreq = num_fee_freqs;
+    beam.n_dir = n_dir;
+    beam.tile_rows = st.tile_rows.as<int>();
+    beam.uniform_row = -1;
+    const FluxBuckets fbs[3] = {
+        {points->num_power_laws, points->num_curved_power_laws, points->num_lists, points->power_law_fds,
+         points->power_law_sis, points->curved_power_law_fds, points->curved_power_law_sis,
+         points->curved_power_law_qs, points->list_fds},
+        {gaussians->num_power_laws, gaussians->num_curved_power_laws, gaussians->num_lists, gaussians->power_law_fds,
+         gaussians->power_law_sis, gaussians->curved_power_law_fds, gaussians->curved_power_law_sis,
+         gaussians->curved_power_law_qs, gaussians->list_fds},
+        {shapelets->num_power_laws, shapelets->num_curved_power_laws, shapelets->num_lists, shapelets->power_law_fds,
+         shapelets->power_law_sis, shapelets->curved_power_law_fds, shapelets->curved_power_law_sis,
+         shapelets->curved_power_law_qs, shapelets->list_fds}};
+    const int counts[3] = {np, ng, ns};
+    int dir_off = 0;
+    for (int ty = 0; ty < 3; ++ty) {
+        if (counts[ty] == 0) continue;
+        SM_TRY(st.coh.reserve(sizeof(double) * 8 * (size_t)counts[ty] * nchan_pad));
+        SM_TRY(launch_coherency(counts[ty], num_freqs, nchan_pad, st.freqs_pad.as<double>(), fbs[ty], beam, false,
+                                st.coh.as<double>(), 0));
+        beam.dir_offset = dir_off;
+        SM_TRY(launch_autos(num_tiles, num_freqs, counts[ty], st.coh.as<double>(), nchan_pad, beam, false, vis_fb,
+                            nullptr, st.accum.as<double>(), false, 0));
+        dir_off += counts[ty];
+    }
+    SM_TRY(launch_autos_demote((int)nvis, st.accum.as<double>(), vis_fb, nullptr, 0));
+    SM_TRY(hipStreamSynchronize(0));
+    return nullptr;
+}
+
+extern "C" const char *sm_analytic_beam(int32_t n_tiles, const uint32_t *delays, const double *gains,
+                                        int32_t n_freqs, const double *freqs, int32_t n_dirs, const double *d_az,
+                                        const double *d_za, JonesF64 *d_out) {
+    if (n_tiles <= 0 || n_freqs <= 0 || n_dirs <= 0) return nullptr;
+    DevBuf dd, dg, df;
+    SM_TRY(dd.reserve(sizeof(unsigned) * 16 * n_tiles));
+    SM_TRY(dg.reserve(sizeof(double) * 32 * n_tiles));
+    SM_TRY(df.reserve(sizeof(double) * n_freqs));
+    std::vector<unsigned> hd(16 * (size_t)n_tiles, 0u);
+    std::vector<double> hg(32 * (size_t)n_tiles, 1.0);
+    if (delays != nullptr) std::memcpy(hd.data(), delays, sizeof(unsigned) * hd.size());
+    if (gains != nullptr) std::memcpy(hg.data(), gains, sizeof(double) * hg.size());
+    SM_TRY(hipMemcpy(dd.ptr, hd.data(), sizeof(unsigned) * hd.size(), hipMemcpyHostToDevice));
+    SM_TRY(hipMemcpy(dg.ptr, hg.data(), sizeof(double) * hg.size(), hipMemcpyHostToDevice));
+    SM_TRY(hipMemcpy(df.ptr, freqs, sizeof(double) * n_freqs, hipMemcpyHostToDevice));
+    SM_TRY(launch_analytic_beam(n_tiles, dd.as<unsigned>(), dg.as<double>(), n_freqs, df.as<double>(), n_dirs, d_az,
+                                d_za, d_out, 0));
+    SM_TRY(hipStreamSynchronize(0));
+    return nullptr;
+}

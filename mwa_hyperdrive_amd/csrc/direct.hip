@@ -1,0 +1,544 @@
+// FP64 direct measurement-equation kernels for gfx950.
+//
+// V_pq(f) += sum_c (J_p S_c(f) J_q^H) e^{i (u L + v M + w N) f/c} [x envelope]
+// (src/model/cpu.rs:337-345,472-487,702-717; src/gpu/model.cu:99-366).
+//
+// Mapping: one lane per baseline, SM_NCH consecutive channels per lane, the component
+// loop serial per lane. Everything that depends only on (component, channel) is
+// wave-uniform: the coherency J S J^H (hoisted out of the baseline loop when every tile
+// shares one beam), LMN, envelope parameters and shapelet coefficients arrive through
+// scalar loads and SGPR operands, so the vector ALUs do only per-baseline work. Across a
+// chunk of evenly spaced channels the phase e^{i theta f_k / c} advances by a fixed
+// rotation, so two FP64 sincos per (baseline, component, chunk) replace SM_NCH of them;
+// the Gaussian envelope exp(q f_k^2) advances by a geometric recurrence in the same way.
+// Blocks are remapped so that all baseline blocks of a channel chunk land on the same XCD
+// and share that XCD's L2 copy of the chunk's coherency rows.
+#include "sm_internal.h"
+
+namespace sm {
+
+__device__ __forceinline__ void load_jones(const JonesF64 *p, double j[8]) {
+    const double *q = reinterpret_cast<const double *>(p);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) j[e] = q[e];
+}
+
+// (A S) B^H in the reference's order (src/model/cpu.rs:343; marlu Jones Mul and h()).
+__device__ __forceinline__ void sandwich(const double a[8], const double s[8], const double b[8], double o[8]) {
+    double t[8];
+    // t = A S
+    t[0] = (a[0] * s[0] - a[1] * s[1]) + (a[2] * s[4] - a[3] * s[5]);
+    t[1] = (a[0] * s[1] + a[1] * s[0]) + (a[2] * s[5] + a[3] * s[4]);
+    t[2] = (a[0] * s[2] - a[1] * s[3]) + (a[2] * s[6] - a[3] * s[7]);
+    t[3] = (a[0] * s[3] + a[1] * s[2]) + (a[2] * s[7] + a[3] * s[6]);
+    t[4] = (a[4] * s[0] - a[5] * s[1]) + (a[6] * s[4] - a[7] * s[5]);
+    t[5] = (a[4] * s[1] + a[5] * s[0]) + (a[6] * s[5] + a[7] * s[4]);
+    t[6] = (a[4] * s[2] - a[5] * s[3]) + (a[6] * s[6] - a[7] * s[7]);
+    t[7] = (a[4] * s[3] + a[5] * s[2]) + (a[6] * s[7] + a[7] * s[6]);
+    // B^H = [conj b00, conj b10; conj b01, conj b11]
+    const double h00r = b[0], h00i = -b[1], h01r = b[4], h01i = -b[5];
+    const double h10r = b[2], h10i = -b[3], h11r = b[6], h11i = -b[7];
+    o[0] = (t[0] * h00r - t[1] * h00i) + (t[2] * h10r - t[3] * h10i);
+    o[1] = (t[0] * h00i + t[1] * h00r) + (t[2] * h10i + t[3] * h10r);
+    o[2] = (t[0] * h01r - t[1] * h01i) + (t[2] * h11r - t[3] * h11i);
+    o[3] = (t[0] * h01i + t[1] * h01r) + (t[2] * h11i + t[3] * h11r);
+    o[4] = (t[4] * h00r - t[5] * h00i) + (t[6] * h10r - t[7] * h10i);
+    o[5] = (t[4] * h00i + t[5] * h00r) + (t[6] * h10i + t[7] * h10r);
+    o[6] = (t[4] * h01r - t[5] * h01i) + (t[6] * h11r - t[7] * h11i);
+    o[7] = (t[4] * h01i + t[5] * h01r) + (t[6] * h11i + t[7] * h11r);
+}
+
+// ------------------------------------------------------------------ coherency table
+// coh[c][f] = J S J^H (one shared beam row) or S; S from the power-law / curved-power-law
+// parameters at 150 MHz or the list table (src/gpu/model.cu:16-28,137-167).
+__global__ void k_coherency(int ncomp, int nchan, int nchan_pad, const double *__restrict__ freqs, FluxBuckets fb,
+                            BeamDesc beam, int apply_beam, double *__restrict__ coh) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)ncomp * nchan_pad) return;
+    const int c = (int)(idx / nchan_pad);
+    const int f = (int)(idx % nchan_pad);
+    double *out = coh + idx * 8;
+    if (f >= nchan) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) out[e] = 0.0;
+        return;
+    }
+    const double freq = freqs[f];
+    double s[8];
+    if (c < fb.n_pl) {
+        double fd[8];
+        load_jones(&fb.pl_fds[c], fd);
+        const double ratio = pow(freq / SM_POWER_LAW_FD_REF_FREQ, fb.pl_sis[c]);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) s[e] = fd[e] * ratio;
+    } else if (c < fb.n_pl + fb.n_cpl) {
+        const int i = c - fb.n_pl;
+        double fd[8];
+        load_jones(&fb.cpl_fds[i], fd);
+        const double ratio = pow(freq / SM_POWER_LAW_FD_REF_FREQ, fb.cpl_sis[i]);
+        const double lt = log(freq / SM_POWER_LAW_FD_REF_FREQ);
+        const double curved = exp(fb.cpl_qs[i] * (lt * lt));
+#pragma unroll
+        for (int e = 0; e < 8; ++e) s[e] = (fd[e] * ratio) * curved;
+    } else {
+        const int i = c - fb.n_pl - fb.n_cpl;
+        load_jones(&fb.list_fds[(size_t)f * fb.n_list + i], s);
+    }
+    if (apply_beam && beam.jones != nullptr) {
+        double j[8], o[8];
+        load_jones(&beam.jones[((size_t)beam.uniform_row * beam.n_ufreq + beam.freq_map[f]) * beam.n_dir + beam.dir_offset + c], j);
+        sandwich(j, s, j, o);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) out[e] = o[e];
+    } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) out[e] = s[e];
+    }
+}
+
+hipError_t launch_coherency(int ncomp, int nchan, int nchan_pad, const double *d_freqs, FluxBuckets fb,
+                            BeamDesc beam, bool apply_beam, double *d_coh, hipStream_t stream) {
+    if (ncomp <= 0) return hipSuccess;
+    const size_t n = (size_t)ncomp * nchan_pad;
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_coherency, dim3(blocks), dim3(256), 0, stream, ncomp, nchan, nchan_pad, d_freqs, fb, beam,
+                       apply_beam ? 1 : 0, d_coh);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ shapelet envelope
+// sum_k i^{(n1+n2) mod 4} c_k B_{n1}(x_pos) B_{n2}(y_pos) with linear interpolation in the
+// basis table (src/model/cpu.rs:633-695, src/gpu/model.cu:39-97). Positions below the
+// table clamp to index 0 with the unclamped fractional weight, as both reference paths do;
+// positions above it clamp to the last interval, whose values are exact zeros.
+__device__ __forceinline__ void shapelet_envelope(double x_pos, double y_pos, int2 range,
+                                                  const ShapeletCoeff *__restrict__ coeffs,
+                                                  const double *__restrict__ sbf, double &er, double &ei) {
+    const double fx = floor(x_pos), fy = floor(y_pos);
+    const int ix = (int)fmin(fmax(fx, 0.0), (double)(SM_SBF_L - 2));
+    const int iy = (int)fmin(fmax(fy, 0.0), (double)(SM_SBF_L - 2));
+    const double wx = x_pos - fx, wy = y_pos - fy;
+    er = 0.0;
+    ei = 0.0;
+    for (int j = 0; j < range.y; ++j) {
+        const ShapeletCoeff cf = coeffs[range.x + j];
+        const int n1 = cf.n1 < SM_SBF_N ? cf.n1 : SM_SBF_N - 1;
+        const int n2 = cf.n2 < SM_SBF_N ? cf.n2 : SM_SBF_N - 1;
+        const double *r1 = sbf + (size_t)n1 * SM_SBF_L + ix;
+        const double *r2 = sbf + (size_t)n2 * SM_SBF_L + iy;
+        const double x_low = r1[0], x_high = r1[1];
+        const double y_low = r2[0], y_high = r2[1];
+        const double u_value = x_low + (x_high - x_low) * wx;
+        const double v_value = y_low + (y_high - y_low) * wy;
+        const double t = (cf.value * u_value) * v_value;
+        switch ((cf.n1 + cf.n2) & 3) {
+        case 0: er += t; break;
+        case 1: ei += t; break;
+        case 2: er -= t; break;
+        default: ei -= t; break;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ direct kernel
+// Logical block (baseline block, channel chunk) from the hardware block id, placing all
+// baseline blocks of a chunk on one XCD (blocks are dealt round-robin to the 8 XCDs).
+__device__ __forceinline__ void logical_block(int gx, int total, int &bx, int &chunk) {
+    int b = blockIdx.x;
+    if ((total & 7) == 0) b = (b & 7) * (total >> 3) + (b >> 3);
+    chunk = b / gx;
+    bx = b - chunk * gx;
+}
+
+template <int KIND, bool PER_TILE, bool EVEN>
+__global__ __launch_bounds__(256) void k_direct(const DirectArgs a, int gx, int total) {
+    int bx, chunk;
+    logical_block(gx, total, bx, chunk);
+    const int bl_raw = bx * 256 + (int)threadIdx.x;
+    const int bl = bl_raw < a.nbl ? bl_raw : a.nbl - 1;
+    const int f0 = chunk * SM_NCH;
+
+    const double *__restrict__ coh = a.coh;
+    const double4 *__restrict__ lmn4 = a.lmn;
+    const double4 *__restrict__ shape = a.shape;
+    const double *__restrict__ freqs = a.freqs;
+
+    const UVW uvw = a.uvw[bl];
+    const double F0 = freqs[f0];
+    const double DF = freqs[f0 + 1] - F0;
+    const double f0c = F0 / SM_VEL_C;
+    const double dfc = DF / SM_VEL_C;
+
+    int2 rows = make_int2(0, 0);
+    if (PER_TILE) rows = a.beam.bl_rows[bl];
+
+    double acc[SM_NCH][8];
+#pragma unroll
+    for (int k = 0; k < SM_NCH; ++k)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) acc[k][e] = 0.0;
+
+    for (int c = 0; c < a.ncomp; ++c) {
+        const double4 lmn = lmn4[c];
+        const double theta = uvw.u * lmn.x + uvw.v * lmn.y + uvw.w * lmn.z;
+        double zr = 1.0, zi = 0.0, rr = 1.0, ri = 0.0;
+        if (EVEN) {
+            sincos(theta * f0c, &zi, &zr);
+            sincos(theta * dfc, &ri, &rr);
+        }
+        double q = 0.0, env = 1.0, er = 1.0, eg = 1.0;
+        double xk = 0.0, yk = 0.0;
+        int2 crange = make_int2(0, 0);
+        if (KIND == KIND_GAUSSIAN) {
+            const double4 g = shape[c];
+            q = g.x * (uvw.u * uvw.u) + g.y * (uvw.u * uvw.v) + g.z * (uvw.v * uvw.v);
+            if (EVEN) {
+                env = exp(q * (F0 * F0));
+                er = exp(q * (DF * (2.0 * F0 + DF)));
+                eg = exp(q * (2.0 * DF * DF));
+            }
+        } else if (KIND == KIND_SHAPELET) {
+            const double4 sh = shape[c];
+            const double2 suv = a.bl_uv[(size_t)c * a.nbl + bl];
+            xk = (suv.x * sh.x + suv.y * sh.y) * sh.z;
+            yk = (suv.x * sh.y - suv.y * sh.x) * sh.w;
+            crange = a.coeff_range[c];
+        }
+        const double *cc = coh + ((size_t)c * a.nchan_pad + f0) * 8;
+#pragma unroll
+        for (int k = 0; k < SM_NCH; ++k) {
+            double pr, pi;
+            if (EVEN) {
+                pr = zr;
+                pi = zi;
+            } else {
+                sincos(theta * (freqs[f0 + k] / SM_VEL_C), &pi, &pr);
+            }
+            if (KIND == KIND_GAUSSIAN) {
+                const double e = EVEN ? env : exp(q * (freqs[f0 + k] * freqs[f0 + k]));
+                pr *= e;
+                pi *= e;
+            } else if (KIND == KIND_SHAPELET) {
+                const double fk = freqs[f0 + k];
+                double sr, si;
+                shapelet_envelope(xk * fk + SM_SBF_C, yk * fk + SM_SBF_C, crange, a.coeffs, a.sbf, sr, si);
+                const double tr = pr * sr - pi * si;
+                const double ti = pr * si + pi * sr;
+                pr = tr;
+                pi = ti;
+            }
+            double cj[8];
+            if (PER_TILE) {
+                double s[8], j1[8], j2[8];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) s[e] = cc[k * 8 + e];
+                const int uf = a.beam.freq_map[f0 + k < a.nchan ? f0 + k : a.nchan - 1];
+                load_jones(&a.beam.jones[((size_t)rows.x * a.beam.n_ufreq + uf) * a.beam.n_dir + a.beam.dir_offset + c], j1);
+                load_jones(&a.beam.jones[((size_t)rows.y * a.beam.n_ufreq + uf) * a.beam.n_dir + a.beam.dir_offset + c], j2);
+                sandwich(j1, s, j2, cj);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) cj[e] = cc[k * 8 + e];
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                acc[k][2 * e] = fma(cj[2 * e], pr, acc[k][2 * e]);
+                acc[k][2 * e] = fma(-cj[2 * e + 1], pi, acc[k][2 * e]);
+                acc[k][2 * e + 1] = fma(cj[2 * e], pi, acc[k][2 * e + 1]);
+                acc[k][2 * e + 1] = fma(cj[2 * e + 1], pr, acc[k][2 * e + 1]);
+            }
+            if (EVEN) {
+                const double nr = zr * rr - zi * ri;
+                zi = zr * ri + zi * rr;
+                zr = nr;
+                if (KIND == KIND_GAUSSIAN) {
+                    env *= er;
+                    er *= eg;
+                }
+            }
+        }
+    }
+    if (bl_raw >= a.nbl) return;
+#pragma unroll
+    for (int k = 0; k < SM_NCH; ++k) {
+        const int f = f0 + k;
+        if (f >= a.nchan) break;
+        const size_t o = (size_t)f * a.nbl + bl;
+        float *v32 = reinterpret_cast<float *>(&a.vis32[o]);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v32[e] += (float)acc[k][e];
+        if (a.vis64 != nullptr) {
+            double *v64 = reinterpret_cast<double *>(&a.vis64[o]);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) v64[e] += acc[k][e];
+        }
+    }
+}
+
+template <int KIND, bool PT>
+static hipError_t launch_direct_kind(const DirectArgs &a, bool even, hipStream_t stream) {
+    const int gx = (a.nbl + 255) / 256;
+    const int gy = (a.nchan + SM_NCH - 1) / SM_NCH;
+    const int total = gx * gy;
+    if (even)
+        hipLaunchKernelGGL((k_direct<KIND, PT, true>), dim3(total), dim3(256), 0, stream, a, gx, total);
+    else
+        hipLaunchKernelGGL((k_direct<KIND, PT, false>), dim3(total), dim3(256), 0, stream, a, gx, total);
+    return hipGetLastError();
+}
+
+// Whether every chunk's channels are exactly f0 + k (f1 - f0); otherwise each channel gets
+// its own sincos/exp. Evaluated on the host copy of the padded frequencies.
+bool chunks_even(const double *freqs, int nchan) {
+    for (int f0 = 0; f0 < nchan; f0 += SM_NCH) {
+        const double d = freqs[f0 + 1] - freqs[f0];
+        for (int k = 2; k < SM_NCH && f0 + k < nchan; ++k)
+            if (freqs[f0 + k] != freqs[f0] + (double)k * d) return false;
+    }
+    return true;
+}
+
+hipError_t launch_direct(Kind kind, bool per_tile, bool even, const DirectArgs &a, hipStream_t stream) {
+    if (a.ncomp <= 0 || a.nbl <= 0 || a.nchan <= 0) return hipSuccess;
+    switch (kind) {
+    case KIND_POINT:
+        return per_tile ? launch_direct_kind<KIND_POINT, true>(a, even, stream)
+                        : launch_direct_kind<KIND_POINT, false>(a, even, stream);
+    case KIND_GAUSSIAN:
+        return per_tile ? launch_direct_kind<KIND_GAUSSIAN, true>(a, even, stream)
+                        : launch_direct_kind<KIND_GAUSSIAN, false>(a, even, stream);
+    default:
+        return per_tile ? launch_direct_kind<KIND_SHAPELET, true>(a, even, stream)
+                        : launch_direct_kind<KIND_SHAPELET, false>(a, even, stream);
+    }
+}
+
+// ------------------------------------------------------------------ autos
+// V_tt(f) += sum_c J_t S_c J_t^H (src/model/cpu.rs:869-904, src/gpu/model.cu:368-485).
+__global__ void k_autos(int ntiles, int nchan, int ncomp, const double *__restrict__ coh, int nchan_pad,
+                        BeamDesc beam, int coh_has_beam, JonesF32 *vis32, JonesF64 *vis64, double *accum64,
+                        int demote_now) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntiles * nchan) return;
+    const int f = i % nchan;
+    const int t = i / nchan;
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int c = 0; c < ncomp; ++c) {
+        const double *s = coh + ((size_t)c * nchan_pad + f) * 8;
+        if (coh_has_beam || beam.jones == nullptr) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) acc[e] += s[e];
+        } else {
+            double j[8], sv[8], o[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) sv[e] = s[e];
+            load_jones(&beam.jones[((size_t)beam.tile_rows[t] * beam.n_ufreq + beam.freq_map[f]) * beam.n_dir + beam.dir_offset + c], j);
+            sandwich(j, sv, j, o);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) acc[e] += o[e];
+        }
+    }
+    const size_t o = (size_t)f * ntiles + t;
+    if (demote_now) {
+        float *v32 = reinterpret_cast<float *>(&vis32[o]);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v32[e] += (float)acc[e];
+        if (vis64 != nullptr) {
+            double *v64 = reinterpret_cast<double *>(&vis64[o]);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) v64[e] += acc[e];
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) accum64[o * 8 + e] += acc[e];
+    }
+}
+
+hipError_t launch_autos(int ntiles, int nchan, int ncomp, const double *d_coh, int nchan_pad, BeamDesc beam,
+                        bool coh_has_beam, JonesF32 *vis32, JonesF64 *vis64, double *accum64, bool demote_now,
+                        hipStream_t stream) {
+    if (ncomp <= 0 || ntiles <= 0 || nchan <= 0) return hipSuccess;
+    const int n = ntiles * nchan;
+    hipLaunchKernelGGL(k_autos, dim3((n + 255) / 256), dim3(256), 0, stream, ntiles, nchan, ncomp, d_coh, nchan_pad,
+                       beam, coh_has_beam ? 1 : 0, vis32, vis64, accum64, demote_now ? 1 : 0);
+    return hipGetLastError();
+}
+
+__global__ void k_autos_demote(int n, const double *__restrict__ accum64, JonesF32 *vis32, JonesF64 *vis64) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float *v32 = reinterpret_cast<float *>(&vis32[i]);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v32[e] += (float)accum64[(size_t)i * 8 + e];
+    if (vis64 != nullptr) {
+        double *v64 = reinterpret_cast<double *>(&vis64[i]);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v64[e] += accum64[(size_t)i * 8 + e];
+    }
+}
+
+hipError_t launch_autos_demote(int n, const double *accum64, JonesF32 *vis32, JonesF64 *vis64, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_autos_demote, dim3((n + 255) / 256), dim3(256), 0, stream, n, accum64, vis32, vis64);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ shapelet UVs
+// Baseline (u, v) with each shapelet component as phase centre: per-tile UVW from the
+// component's (sin ha, cos ha, sin dec, cos dec), then tile1 - tile2
+// (src/srclist/types/components/mod.rs:299-315; coords.md "UVWs"). Contraction is off so
+// the values equal the CPU restatement bit for bit.
+__global__ void k_shapelet_bl_uv(int ncomp, int ntiles, int nbl, const double *__restrict__ hadec,
+                                 const double *__restrict__ xyz, const int2 *__restrict__ bl_tiles,
+                                 double2 *__restrict__ out) {
+#pragma clang fp contract(off)
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)ncomp * nbl) return;
+    const int c = (int)(idx / nbl);
+    const int b = (int)(idx % nbl);
+    const double s_ha = hadec[4 * c], c_ha = hadec[4 * c + 1], s_dec = hadec[4 * c + 2];
+    const int2 t = bl_tiles[b];
+    const double x1 = xyz[3 * t.x], y1 = xyz[3 * t.x + 1], z1 = xyz[3 * t.x + 2];
+    const double x2 = xyz[3 * t.y], y2 = xyz[3 * t.y + 1], z2 = xyz[3 * t.y + 2];
+    const double c_dec = hadec[4 * c + 3];
+    const double u1 = s_ha * x1 + c_ha * y1;
+    const double v1 = -s_dec * c_ha * x1 + s_dec * s_ha * y1 + c_dec * z1;
+    const double u2 = s_ha * x2 + c_ha * y2;
+    const double v2 = -s_dec * c_ha * x2 + s_dec * s_ha * y2 + c_dec * z2;
+    (void)ntiles;
+    out[idx] = make_double2(u1 - u2, v1 - v2);
+}
+
+hipError_t launch_shapelet_bl_uv(int ncomp, int ntiles, int nbl, const double *d_comp_hadec, const double *d_xyz,
+                                 const int2 *d_bl_tiles, double2 *d_bl_uv, hipStream_t stream) {
+    if (ncomp <= 0 || nbl <= 0) return hipSuccess;
+    const size_t n = (size_t)ncomp * nbl;
+    hipLaunchKernelGGL(k_shapelet_bl_uv, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, ncomp, ntiles, nbl,
+                       d_comp_hadec, d_xyz, d_bl_tiles, d_bl_uv);
+    return hipGetLastError();
+}
+
+// [baseline][bucket component] (three buckets) -> [component][baseline].
+__global__ void k_repack_uv(int nbl, int n_pl, int n_cpl, int n_list, const ShapeletUV *__restrict__ pl,
+                            const ShapeletUV *__restrict__ cpl, const ShapeletUV *__restrict__ list,
+                            double2 *__restrict__ out) {
+    const int n = n_pl + n_cpl + n_list;
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)n * nbl) return;
+    const int c = (int)(idx / nbl);
+    const int b = (int)(idx % nbl);
+    ShapeletUV s;
+    if (c < n_pl)
+        s = pl[(size_t)b * n_pl + c];
+    else if (c < n_pl + n_cpl)
+        s = cpl[(size_t)b * n_cpl + (c - n_pl)];
+    else
+        s = list[(size_t)b * n_list + (c - n_pl - n_cpl)];
+    out[idx] = make_double2(s.u, s.v);
+}
+
+hipError_t launch_repack_shapelet_uv(int nbl, int n_pl, int n_cpl, int n_list, const ShapeletUV *pl,
+                                     const ShapeletUV *cpl, const ShapeletUV *list, double2 *d_bl_uv,
+                                     hipStream_t stream) {
+    const size_t n = (size_t)(n_pl + n_cpl + n_list) * nbl;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_repack_uv, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, nbl, n_pl, n_cpl, n_list,
+                       pl, cpl, list, d_bl_uv);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ analytic beam
+// The product's MWA-like beam (see the header comment of sm_analytic_beam): 4x4 crossed
+// short dipoles, 1.1 m pitch, 0.278 m above a ground plane, 435 ps delay steps, delay 32 =
+// dead dipole. The dipole phase factorises into column x row terms, so a direction needs
+// 4 sincos for geometry; the 16 delay phasors of a (tile, freq) are built once per block.
+__global__ void k_analytic_beam(int n_freqs, const double *__restrict__ freqs, int n_dirs,
+                                const double *__restrict__ az, const double *__restrict__ za,
+                                const unsigned *__restrict__ delays, const double *__restrict__ gains,
+                                JonesF64 *__restrict__ out) {
+    __shared__ double s_wx[16][2], s_wy[16][2];  // gain * delay phasor per dipole (X, Y)
+    const int tf = blockIdx.y;                  // tile * n_freqs + freq
+    const int tile = tf / n_freqs;
+    const int fi = tf - tile * n_freqs;
+    const double freq = freqs[fi];
+    if (threadIdx.x < 16) {
+        const int d = threadIdx.x;
+        unsigned dl = delays[16 * tile + d];
+        double gx = gains[32 * tile + d], gy = gains[32 * tile + 16 + d];
+        if (dl == 32) {
+            gx = 0.0;
+            gy = 0.0;
+            dl = 0;
+        }
+        double s, c;
+        sincos(-6.283185307179586 * freq * ((double)dl * 435e-12), &s, &c);
+        s_wx[d][0] = gx * c;
+        s_wx[d][1] = gx * s;
+        s_wy[d][0] = gy * c;
+        s_wy[d][1] = gy * s;
+    }
+    __syncthreads();
+    const int dir = blockIdx.x * blockDim.x + threadIdx.x;
+    if (dir >= n_dirs) return;
+    double s_za, c_za, s_az, c_az;
+    sincos(za[dir], &s_za, &c_za);
+    sincos(az[dir], &s_az, &c_az);
+    const double r_e = s_za * s_az, r_n = s_za * c_az, r_u = c_za;
+    const double k = 6.283185307179586 * freq / SM_VEL_C;
+    // Column phases exp(i k x r_e) for x = -1.65, -0.55, 0.55, 1.65 m; row phases for y.
+    double cx[4][2], cy[4][2];
+    {
+        double s1, c1, s3, c3;
+        sincos(k * 0.55 * r_e, &s1, &c1);
+        sincos(k * 1.65 * r_e, &s3, &c3);
+        cx[0][0] = c3; cx[0][1] = -s3;
+        cx[1][0] = c1; cx[1][1] = -s1;
+        cx[2][0] = c1; cx[2][1] = s1;
+        cx[3][0] = c3; cx[3][1] = s3;
+        sincos(k * 0.55 * r_n, &s1, &c1);
+        sincos(k * 1.65 * r_n, &s3, &c3);
+        // rows 0..3 have y = +1.65, +0.55, -0.55, -1.65 m
+        cy[0][0] = c3; cy[0][1] = s3;
+        cy[1][0] = c1; cy[1][1] = s1;
+        cy[2][0] = c1; cy[2][1] = -s1;
+        cy[3][0] = c3; cy[3][1] = -s3;
+    }
+    double ax_r = 0, ax_i = 0, ay_r = 0, ay_i = 0;
+#pragma unroll
+    for (int d = 0; d < 16; ++d) {
+        const int col = d & 3, row = d >> 2;
+        const double pr = cx[col][0] * cy[row][0] - cx[col][1] * cy[row][1];
+        const double pi = cx[col][0] * cy[row][1] + cx[col][1] * cy[row][0];
+        ax_r += s_wx[d][0] * pr - s_wx[d][1] * pi;
+        ax_i += s_wx[d][0] * pi + s_wx[d][1] * pr;
+        ay_r += s_wy[d][0] * pr - s_wy[d][1] * pi;
+        ay_i += s_wy[d][0] * pi + s_wy[d][1] * pr;
+    }
+    const double g = sin(k * 0.278 * r_u) / (16.0 * sin(k * 0.278));
+    double sx = sqrt(1.0 - r_e * r_e);
+    if (sx < 1e-12) sx = 1e-12;
+    const double px = g * sx, pyx = g * (-r_e * r_n / sx), pyy = g * (r_u / sx);
+    JonesF64 j;
+    j.j00_re = ax_r * px;
+    j.j00_im = ax_i * px;
+    j.j01_re = 0.0;
+    j.j01_im = 0.0;
+    j.j10_re = ay_r * pyx;
+    j.j10_im = ay_i * pyx;
+    j.j11_re = ay_r * pyy;
+    j.j11_im = ay_i * pyy;
+    out[(size_t)tf * n_dirs + dir] = j;
+}
+
+hipError_t launch_analytic_beam(int n_tiles, const unsigned *d_delays, const double *d_gains, int n_freqs,
+                                const double *d_freqs, int n_dirs, const double *d_az, const double *d_za,
+                                JonesF64 *d_out, hipStream_t stream) {
+    if (n_tiles <= 0 || n_freqs <= 0 || n_dirs <= 0) return hipSuccess;
+    if ((long)n_tiles * n_freqs > 65535) return hipErrorInvalidValue;
+    dim3 grid((n_dirs + 255) / 256, n_tiles * n_freqs);
+    hipLaunchKernelGGL(k_analytic_beam, grid, dim3(256), 0, stream, n_freqs, d_freqs, n_dirs, d_az, d_za, d_delays,
+                       d_gains, d_out);
+    return hipGetLastError();
+}
+
+}  // namespace sm

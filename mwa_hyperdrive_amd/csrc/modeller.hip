@@ -1,0 +1,654 @@
+// SkyModeller handle: the host side of the predictor (src/model/gpu.rs:150-1527 and
+// src/model/cpu.rs:73-180 re-expressed in C++ over the HIP kernels).
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "sm_abi_common.h"
+
+namespace sm {
+namespace {
+
+constexpr double kTau = 6.283185307179586;
+constexpr double kDefaultSpecIndex = -0.8;  // src/constants.rs:8
+
+struct Cplx {
+    double re, im;
+};
+
+// --------------------------------------------------------------------- flux estimation
+// FluxDensityType::estimate_at_freq (src/srclist/types/flux_density/mod.rs:158-270).
+SmFluxDensity estimate_at_freq(const SmComponent &c, double freq) {
+    auto scaled = [](SmFluxDensity fd, double r) {
+        fd.i *= r;
+        fd.q *= r;
+        fd.u *= r;
+        fd.v *= r;
+        return fd;
+    };
+    if (c.flux_type == SM_POWER_LAW) {
+        SmFluxDensity out = scaled(c.fds[0], std::pow(freq / c.fds[0].freq, c.si));
+        out.freq = freq;
+        return out;
+    }
+    if (c.flux_type == SM_CURVED_POWER_LAW) {
+        SmFluxDensity pl = scaled(c.fds[0], std::pow(freq / c.fds[0].freq, c.si));
+        pl.freq = freq;
+        const double lr = std::log(freq / c.fds[0].freq);
+        return scaled(pl, std::exp(c.q * (lr * lr)));
+    }
+    const SmFluxDensity *fds = c.fds;
+    const int n = c.num_fds;
+    double si;
+    const SmFluxDensity *base;
+    if (n == 1) {
+        si = kDefaultSpecIndex;
+        base = &fds[0];
+    } else {
+        const SmFluxDensity *a = &fds[0], *b = &fds[1];
+        for (int w = 0; w + 1 < n; ++w) {
+            a = &fds[w];
+            b = &fds[w + 1];
+            if (std::fabs(a->freq - freq) < 1e-3) return *a;
+            if (std::fabs(b->freq - freq) < 1e-3) return *b;
+            if (freq < b->freq) break;
+        }
+        const bool sa = std::signbit(a->i), sb = std::signbit(b->i);
+        if (sa != sb) {
+            const double run = b->freq - a->freq;
+            SmFluxDensity out;
+            out.freq = freq;
+            out.i = a->i + (b->i - a->i) / run * (freq - a->freq);
+            out.q = a->q + (b->q - a->q) / run * (freq - a->freq);
+            out.u = a->u + (b->u - a->u) / run * (freq - a->freq);
+            out.v = a->v + (b->v - a->v) / run * (freq - a->freq);
+            return out;
+        }
+        si = std::log(b->i / a->i) / std::log(b->freq / a->freq);
+        base = b->freq < freq ? b : a;
+    }
+    SmFluxDensity out = scaled(*base, std::pow(freq / base->freq, si));
+    out.freq = freq;
+    return out;
+}
+
+// FluxDensity::to_inst_stokes (flux_density/mod.rs:58-65): XX=I-Q, XY=U-iV, YX=U+iV, YY=I+Q.
+JonesF64 inst_stokes(const SmFluxDensity &fd) {
+    JonesF64 j;
+    j.j00_re = fd.i - fd.q;
+    j.j00_im = 0.0;
+    j.j01_re = fd.u;
+    j.j01_im = -fd.v;
+    j.j10_re = fd.u;
+    j.j10_im = fd.v;
+    j.j11_re = fd.i + fd.q;
+    j.j11_im = 0.0;
+    return j;
+}
+
+// ------------------------------------------------------------------------- geometry
+// RADec::to_lmn(..).prepare_for_rime() (marlu; src/srclist/types/components/mod.rs:214).
+double4 lmn_rime(double ra, double dec, double pc_ra, double pc_dec) {
+    const double d_ra = ra - pc_ra;
+    const double l = std::cos(dec) * std::sin(d_ra);
+    const double m = std::sin(dec) * std::cos(pc_dec) - std::cos(dec) * std::sin(pc_dec) * std::cos(d_ra);
+    const double n = std::sin(dec) * std::sin(pc_dec) + std::cos(dec) * std::cos(pc_dec) * std::cos(d_ra);
+    return make_double4(kTau * l, kTau * m, kTau * (n - 1.0), 0.0);
+}
+
+// Baseline UVWs, antenna1 - antenna2 over the upper triangle (mdbook/src/defs/coords.md).
+void cross_uvws(const std::vector<double> &xyz, int nt, double ha, double dec, std::vector<UVW> &out) {
+    const double s_ha = std::sin(ha), c_ha = std::cos(ha), s_dec = std::sin(dec), c_dec = std::cos(dec);
+    std::vector<UVW> tile(nt);
+    for (int i = 0; i < nt; ++i) {
+        const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+        tile[i].u = s_ha * x + c_ha * y;
+        tile[i].v = -s_dec * c_ha * x + s_dec * s_ha * y + c_dec * z;
+        tile[i].w = c_dec * c_ha * x - c_dec * s_ha * y + s_dec * z;
+    }
+    out.resize((size_t)nt * (nt - 1) / 2);
+    size_t b = 0;
+    for (int i = 0; i < nt; ++i)
+        for (int j = i + 1; j < nt; ++j, ++b) {
+            out[b].u = tile[i].u - tile[j].u;
+            out[b].v = tile[i].v - tile[j].v;
+            out[b].w = tile[i].w - tile[j].w;
+        }
+}
+
+// HADec::to_azel (ERFA hd2ae formulation), returns (az, za).
+void azza(double ha, double dec, double lat, double &az, double &za) {
+    const double s_ha = std::sin(ha), c_ha = std::cos(ha), s_dec = std::sin(dec), c_dec = std::cos(dec);
+    const double s_lat = std::sin(lat), c_lat = std::cos(lat);
+    const double x = -c_ha * c_dec * s_lat + s_dec * c_lat;
+    const double y = -s_ha * c_dec;
+    const double z = c_ha * c_dec * c_lat + s_dec * s_lat;
+    const double r = std::hypot(x, y);
+    const double a = r != 0.0 ? std::atan2(y, x) : 0.0;
+    az = a < 0.0 ? a + kTau : a;
+    za = 1.5707963267948966 - std::atan2(z, r);
+}
+
+// Hermite-function table B_n(x_j), x_j = j/100 - 50 (src/model/make_shapelet_basis_values.py),
+// by the normalised recurrence B_{n+1} = sqrt(2/(n+1)) x B_n - sqrt(n/(n+1)) B_{n-1}.
+std::vector<double> shapelet_table() {
+    std::vector<double> t((size_t)SM_SBF_N * SM_SBF_L);
+    for (int j = 0; j < SM_SBF_L; ++j) {
+        const double x = j == SM_SBF_L - 1 ? 50.0 : (double)j * (100.0 / 10000.0) + -50.0;
+        double prev = 0.0, cur = std::exp(-0.5 * (x * x));
+        t[j] = cur;
+        for (int n = 0; n + 1 < SM_SBF_N; ++n) {
+            const double next = std::sqrt(2.0 / (n + 1)) * x * cur - std::sqrt((double)n / (n + 1)) * prev;
+            prev = cur;
+            cur = next;
+            t[(size_t)(n + 1) * SM_SBF_L + j] = cur;
+        }
+    }
+    return t;
+}
+
+// --------------------------------------------------------------------- component lists
+struct HostType {
+    int n_pl = 0, n_cpl = 0, n_list = 0;
+    std::vector<RADec> radec;  // concatenated PL, CPL, list
+    std::vector<double4> lmn, shape;
+    std::vector<JonesF64> pl_fds, cpl_fds, list_fds;  // list: [nchan][n_list]
+    std::vector<double> pl_sis, cpl_sis, cpl_qs;
+    std::vector<ShapeletCoeff> coeffs;
+    std::vector<int2> ranges;
+    int n() const { return n_pl + n_cpl + n_list; }
+};
+
+struct DevType {
+    DevBuf lmn, shape, pl_fds, pl_sis, cpl_fds, cpl_sis, cpl_qs, list_fds, coeffs, ranges;
+    DevBuf bl_uv, hadec, az, za, beam, coh;
+    HostType h;
+};
+
+template <class T> hipError_t upload(DevBuf &b, const std::vector<T> &v) {
+    if (v.empty()) return hipSuccess;
+    hipError_t e = b.reserve(sizeof(T) * v.size());
+    if (e != hipSuccess) return e;
+    return hipMemcpy(b.ptr, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice);
+}
+
+}  // namespace
+}  // namespace sm
+
+using namespace sm;
+
+struct SmModeller {
+    // configuration
+    int nt = 0, n_total = 0, nbl = 0, nchan = 0, nchan_pad = 0;
+    std::vector<double> xyz, freqs, freqs_pad;
+    std::vector<int> flagged;
+    double pc_ra = 0, pc_dec = 0, lon = 0, lat = 0;
+    int beam_type = SM_BEAM_NONE, pols = SM_POLS_XX_XY_YX_YY;
+    std::vector<unsigned> delays;  // [n_total][16]
+    std::vector<double> gains;     // [n_total][32]
+    // unique tiles / freqs (src/model/cpu.rs:105-161)
+    int n_utile = 1, n_ufreq = 1;
+    std::vector<int> tile_rows;  // unflagged tile -> unique tile row
+    std::vector<int> unique_tile_total;  // unique row -> total-array tile
+    std::vector<int> freq_map;
+    std::vector<double> ufreqs;
+    bool even = false;
+    // device
+    DevBuf d_freqs, d_xyz, d_bl_tiles, d_bl_rows, d_tile_rows, d_freq_map, d_ufreqs, d_delays, d_gains, d_sbf;
+    DevBuf d_uvw, d_vis32, d_vis64, d_accum;
+    DevType types[3];
+    // timing
+    bool timing = false;
+    hipEvent_t ev[3][2] = {};
+    int launches = 0;
+    double ms_total = 0, ms_type[3] = {0, 0, 0};
+    std::vector<UVW> h_uvw;
+};
+
+namespace {
+
+bool tile_is_flagged(const SmModeller *m, int t) {
+    for (int f : m->flagged)
+        if (f == t) return true;
+    return false;
+}
+
+// Unique tiles by (delays, gains with delay 32 -> gain 0) (src/model/cpu.rs:105-143,944-964).
+void unique_tiles(SmModeller *m) {
+    m->tile_rows.assign(m->nt, 0);
+    m->unique_tile_total.clear();
+    if (m->beam_type == SM_BEAM_NONE) {
+        m->n_utile = 1;
+        m->unique_tile_total.push_back(0);
+        return;
+    }
+    std::vector<std::vector<double>> keys;
+    int u = 0;
+    for (int t = 0; t < m->n_total; ++t) {
+        if (tile_is_flagged(m, t)) continue;
+        std::vector<double> key(48);
+        for (int k = 0; k < 16; ++k) key[k] = (double)m->delays[16 * t + k];
+        for (int k = 0; k < 32; ++k) key[16 + k] = m->delays[16 * t + (k % 16)] == 32 ? 0.0 : m->gains[32 * t + k];
+        int found = -1;
+        for (size_t i = 0; i < keys.size() && found < 0; ++i)
+            if (std::memcmp(keys[i].data(), key.data(), sizeof(double) * 48) == 0) found = (int)i;
+        if (found < 0) {
+            found = (int)keys.size();
+            keys.push_back(key);
+            m->unique_tile_total.push_back(t);
+        }
+        m->tile_rows[u++] = found;
+    }
+    m->n_utile = (int)keys.size();
+}
+
+const char *validate_component(const SmComponent &c) {
+    if (c.comp_type < SM_POINT || c.comp_type > SM_SHAPELET) return "unknown component type";
+    if (c.flux_type < SM_LIST || c.flux_type > SM_CURVED_POWER_LAW) return "unknown flux-density type";
+    if (c.fds == nullptr || c.num_fds < 1) return "a component needs at least one flux density";
+    if (c.flux_type == SM_LIST)
+        for (int i = 1; i < c.num_fds; ++i)
+            if (c.fds[i].freq < c.fds[i - 1].freq) return "list flux densities are not sorted by frequency";
+    if (c.comp_type == SM_SHAPELET) {
+        if (c.num_coeffs < 0 || (c.num_coeffs > 0 && c.coeffs == nullptr)) return "bad shapelet coefficients";
+        for (int i = 0; i < c.num_coeffs; ++i)
+            if (c.coeffs[i].n1 >= SM_SBF_N || c.coeffs[i].n2 >= SM_SBF_N)
+                return "shapelet coefficient order exceeds the basis table (n1, n2 <= 100)";
+    }
+    return nullptr;
+}
+
+// ComponentList / SkyModellerGpu::update_source_list (src/model/gpu.rs:521-861): nine buckets
+// concatenated per type as power law, curved power law, list.
+const char *set_components(SmModeller *m, const SmComponent *const *comps, int ncomp) {
+    for (int i = 0; i < ncomp; ++i)
+        if (const char *e = validate_component(*comps[i])) return e;
+    for (int ty = 0; ty < 3; ++ty) {
+        HostType h;
+        for (int pass = 0; pass < 3; ++pass) {
+            const int ft = pass == 0 ? SM_POWER_LAW : (pass == 1 ? SM_CURVED_POWER_LAW : SM_LIST);
+            for (int i = 0; i < ncomp; ++i) {
+                const SmComponent &c = *comps[i];
+                if (c.comp_type != ty || c.flux_type != ft) continue;
+                h.radec.push_back(RADec{c.ra, c.dec});
+                h.lmn.push_back(lmn_rime(c.ra, c.dec, m->pc_ra, m->pc_dec));
+                GaussianParams gp{c.maj, c.min, c.pa};
+                if (ty == SM_GAUSSIAN) h.shape.push_back(host_gaussian_shape(gp));
+                if (ty == SM_SHAPELET) {
+                    h.shape.push_back(host_shapelet_shape(gp));
+                    h.ranges.push_back(make_int2((int)h.coeffs.size(), c.num_coeffs));
+                    for (int k = 0; k < c.num_coeffs; ++k) h.coeffs.push_back(c.coeffs[k]);
+                }
+                if (ft == SM_POWER_LAW) {
+                    h.pl_fds.push_back(inst_stokes(estimate_at_freq(c, SM_POWER_LAW_FD_REF_FREQ)));
+                    h.pl_sis.push_back(c.si);
+                    h.n_pl++;
+                } else if (ft == SM_CURVED_POWER_LAW) {
+                    const SmFluxDensity at150 = estimate_at_freq(c, SM_POWER_LAW_FD_REF_FREQ);
+                    h.cpl_fds.push_back(inst_stokes(at150));
+                    // Re-referenced spectral index (src/model/gpu.rs:664-680).
+                    double si = c.si;
+                    if (c.fds[0].freq != SM_POWER_LAW_FD_REF_FREQ) {
+                        const double lr = std::log(c.fds[0].freq / SM_POWER_LAW_FD_REF_FREQ);
+                        si = (std::log(c.fds[0].i / at150.i) - c.q * (lr * lr)) / lr;
+                    }
+                    h.cpl_sis.push_back(si);
+                    h.cpl_qs.push_back(c.q);
+                    h.n_cpl++;
+                } else {
+                    h.n_list++;
+                }
+            }
+        }
+        // List flux densities [freq][component] (src/model/gpu.rs:768-774).
+        if (h.n_list > 0) {
+            std::vector<const SmComponent *> lists;
+            for (int i = 0; i < ncomp; ++i)
+                if (comps[i]->comp_type == ty && comps[i]->flux_type == SM_LIST) lists.push_back(comps[i]);
+            h.list_fds.resize((size_t)m->nchan * h.n_list);
+            for (int f = 0; f < m->nchan; ++f)
+                for (int i = 0; i < h.n_list; ++i)
+                    h.list_fds[(size_t)f * h.n_list + i] = inst_stokes(estimate_at_freq(*lists[i], m->freqs[f]));
+        }
+        DevType &d = m->types[ty];
+        SM_TRY(upload(d.lmn, h.lmn));
+        SM_TRY(upload(d.shape, h.shape));
+        SM_TRY(upload(d.pl_fds, h.pl_fds));
+        SM_TRY(upload(d.pl_sis, h.pl_sis));
+        SM_TRY(upload(d.cpl_fds, h.cpl_fds));
+        SM_TRY(upload(d.cpl_sis, h.cpl_sis));
+        SM_TRY(upload(d.cpl_qs, h.cpl_qs));
+        SM_TRY(upload(d.list_fds, h.list_fds));
+        SM_TRY(upload(d.coeffs, h.coeffs));
+        SM_TRY(upload(d.ranges, h.ranges));
+        d.h = std::move(h);
+    }
+    return nullptr;
+}
+
+TypeInputs type_inputs(DevType &d) {
+    const HostType &h = d.h;
+    TypeInputs in{};
+    in.fb = FluxBuckets{h.n_pl,
+                        h.n_cpl,
+                        h.n_list,
+                        d.pl_fds.as<JonesF64>(),
+                        d.pl_sis.as<double>(),
+                        d.cpl_fds.as<JonesF64>(),
+                        d.cpl_sis.as<double>(),
+                        d.cpl_qs.as<double>(),
+                        d.list_fds.as<JonesF64>()};
+    in.lmn = d.lmn.as<double4>();
+    in.shape = d.shape.as<double4>();
+    in.bl_uv = d.bl_uv.as<double2>();
+    in.coeff_range = d.ranges.as<int2>();
+    in.coeffs = d.coeffs.as<ShapeletCoeff>();
+    return in;
+}
+
+// Beam responses [unique tile][unique freq][component] for one type at this LST
+// (src/model/gpu.rs:884-911) and the matching BeamDesc.
+const char *prepare_beam(SmModeller *m, DevType &d, double lst, double latitude, BeamDesc &b) {
+    const int n = d.h.n();
+    b = BeamDesc{};
+    b.tile_rows = m->d_tile_rows.as<int>();
+    b.freq_map = m->d_freq_map.as<int>();
+    b.n_ufreq = m->n_ufreq;
+    b.n_dir = n;
+    b.dir_offset = 0;
+    b.uniform_row = 0;
+    if (m->beam_type == SM_BEAM_NONE || n == 0) return nullptr;
+    std::vector<double> az(n), za(n);
+    for (int i = 0; i < n; ++i) azza(lst - d.h.radec[i].ra, d.h.radec[i].dec, latitude, az[i], za[i]);
+    SM_TRY(upload(d.az, az));
+    SM_TRY(upload(d.za, za));
+    SM_TRY(d.beam.reserve(sizeof(JonesF64) * (size_t)m->n_utile * m->n_ufreq * n));
+    // One launch per unique tile keeps the grid's y extent at n_ufreq.
+    for (int u = 0; u < m->n_utile; ++u) {
+        const int t = m->unique_tile_total[u];
+        SM_TRY(launch_analytic_beam(1, m->d_delays.as<unsigned>() + 16 * t, m->d_gains.as<double>() + 32 * t,
+                                    m->n_ufreq, m->d_ufreqs.as<double>(), n, d.az.as<double>(), d.za.as<double>(),
+                                    d.beam.as<JonesF64>() + (size_t)u * m->n_ufreq * n, 0));
+    }
+    b.jones = d.beam.as<JonesF64>();
+    if (m->n_utile == 1) {
+        b.uniform_row = 0;
+    } else {
+        b.uniform_row = -1;
+        b.bl_rows = m->d_bl_rows.as<int2>();
+    }
+    return nullptr;
+}
+
+const char *prepare_shapelet_uvs(SmModeller *m, DevType &d, double lst) {
+    const int n = d.h.n();
+    if (n == 0) return nullptr;
+    std::vector<double> hd(4 * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+        const double ha = lst - d.h.radec[i].ra, dec = d.h.radec[i].dec;
+        hd[4 * i] = std::sin(ha);
+        hd[4 * i + 1] = std::cos(ha);
+        hd[4 * i + 2] = std::sin(dec);
+        hd[4 * i + 3] = std::cos(dec);
+    }
+    SM_TRY(upload(d.hadec, hd));
+    SM_TRY(d.bl_uv.reserve(sizeof(double2) * (size_t)n * m->nbl));
+    SM_TRY(launch_shapelet_bl_uv(n, m->nt, m->nbl, d.hadec.as<double>(), m->d_xyz.as<double>(),
+                                 m->d_bl_tiles.as<int2>(), d.bl_uv.as<double2>(), 0));
+    return nullptr;
+}
+
+void mask_pols(int pols, JonesF32 *v, JonesF64 *v64, size_t n) {
+    if (pols == SM_POLS_XX_XY_YX_YY) return;
+    bool keep[4] = {false, false, false, false};
+    if (pols == SM_POLS_XX) keep[0] = true;
+    if (pols == SM_POLS_YY) keep[3] = true;
+    if (pols == SM_POLS_XX_YY) keep[0] = keep[3] = true;
+    if (pols == SM_POLS_XX_YY_XY) keep[0] = keep[1] = keep[3] = true;
+    for (size_t i = 0; i < n; ++i) {
+        float *f = reinterpret_cast<float *>(&v[i]);
+        double *d = v64 ? reinterpret_cast<double *>(&v64[i]) : nullptr;
+        for (int k = 0; k < 4; ++k)
+            if (!keep[k]) {
+                f[2 * k] = f[2 * k + 1] = 0.0f;
+                if (d) d[2 * k] = d[2 * k + 1] = 0.0;
+            }
+    }
+}
+
+}  // namespace
+
+extern "C" const char *sm_new(const SmObsParams *p, const SmComponent *comps, const int32_t *source_offsets,
+                              int32_t num_sources, SmModeller **out) {
+    if (p == nullptr || out == nullptr) return "null argument";
+    if (p->num_unflagged_tiles < 1) return "need at least one unflagged tile";
+    if (p->num_freqs < 1) return "need at least one frequency";
+    if (p->unflagged_tile_xyzs == nullptr || p->freqs == nullptr) return "null tile positions or frequencies";
+    if (num_sources < 0 || (num_sources > 0 && (source_offsets == nullptr || comps == nullptr)))
+        return "bad source list";
+    if (p->beam_type != SM_BEAM_NONE && p->beam_type != SM_BEAM_ANALYTIC) return "unknown beam type";
+    SmModeller *m = new (std::nothrow) SmModeller();
+    if (m == nullptr) return "out of host memory";
+    m->nt = p->num_unflagged_tiles;
+    m->n_total = p->num_unflagged_tiles + p->num_flagged_tiles;
+    m->nbl = m->nt * (m->nt - 1) / 2;
+    m->nchan = p->num_freqs;
+    m->nchan_pad = pad_channels(m->nchan);
+    m->xyz.assign(p->unflagged_tile_xyzs, p->unflagged_tile_xyzs + 3 * (size_t)m->nt);
+    m->freqs.assign(p->freqs, p->freqs + m->nchan);
+    m->freqs_pad.resize(m->nchan_pad);
+    pad_freqs(m->freqs.data(), m->nchan, m->freqs_pad.data());
+    m->even = chunks_even(m->freqs_pad.data(), m->nchan);
+    if (p->num_flagged_tiles > 0) m->flagged.assign(p->flagged_tiles, p->flagged_tiles + p->num_flagged_tiles);
+    m->pc_ra = p->phase_centre_ra;
+    m->pc_dec = p->phase_centre_dec;
+    m->lon = p->array_longitude;
+    m->lat = p->array_latitude;
+    m->beam_type = p->beam_type;
+    m->pols = p->pols;
+    m->delays.assign((size_t)m->n_total * 16, 0u);
+    m->gains.assign((size_t)m->n_total * 32, 1.0);
+    if (p->dipole_delays) std::memcpy(m->delays.data(), p->dipole_delays, sizeof(unsigned) * m->delays.size());
+    if (p->dipole_gains) std::memcpy(m->gains.data(), p->dipole_gains, sizeof(double) * m->gains.size());
+    unique_tiles(m);
+    // Unique beam frequencies: the analytic beam is defined at every frequency, so only exact
+    // duplicates are merged (src/model/cpu.rs:145-161).
+    m->freq_map.resize(m->nchan);
+    for (int f = 0; f < m->nchan; ++f) {
+        int idx = -1;
+        for (size_t u = 0; u < m->ufreqs.size(); ++u)
+            if (m->ufreqs[u] == m->freqs[f]) idx = (int)u;
+        if (idx < 0) {
+            idx = (int)m->ufreqs.size();
+            m->ufreqs.push_back(m->freqs[f]);
+        }
+        m->freq_map[f] = idx;
+    }
+    m->n_ufreq = (int)m->ufreqs.size();
+    std::vector<int2> bl_tiles(m->nbl), bl_rows(m->nbl);
+    for (int i = 0, b = 0; i < m->nt; ++i)
+        for (int j = i + 1; j < m->nt; ++j, ++b) {
+            bl_tiles[b] = make_int2(i, j);
+            bl_rows[b] = make_int2(m->tile_rows[i], m->tile_rows[j]);
+        }
+    auto fail = [m](const char *e) {
+        sm_free(m);
+        return e;
+    };
+#define SM_NEW_TRY(call)                                        \
+    do {                                                        \
+        hipError_t e_ = (call);                                 \
+        if (e_ != hipSuccess) return fail(hipGetErrorString(e_)); \
+    } while (0)
+    SM_NEW_TRY(upload(m->d_freqs, m->freqs_pad));
+    SM_NEW_TRY(upload(m->d_xyz, m->xyz));
+    SM_NEW_TRY(upload(m->d_bl_tiles, bl_tiles));
+    SM_NEW_TRY(upload(m->d_bl_rows, bl_rows));
+    SM_NEW_TRY(upload(m->d_tile_rows, m->tile_rows));
+    SM_NEW_TRY(upload(m->d_freq_map, m->freq_map));
+    SM_NEW_TRY(upload(m->d_ufreqs, m->ufreqs));
+    SM_NEW_TRY(upload(m->d_delays, m->delays));
+    SM_NEW_TRY(upload(m->d_gains, m->gains));
+    SM_NEW_TRY(upload(m->d_sbf, shapelet_table()));
+    for (int ty = 0; ty < 3; ++ty)
+        for (int k = 0; k < 2; ++k) SM_NEW_TRY(hipEventCreate(&m->ev[ty][k]));
+#undef SM_NEW_TRY
+    // Reverse the source order (src/model/gpu.rs:230-241, src/model/cpu.rs:86-93).
+    std::vector<const SmComponent *> order;
+    for (int s = num_sources - 1; s >= 0; --s)
+        for (int i = source_offsets[s]; i < source_offsets[s + 1]; ++i) order.push_back(&comps[i]);
+    if (const char *e = set_components(m, order.data(), (int)order.size())) return fail(e);
+    *out = m;
+    return nullptr;
+}
+
+extern "C" void sm_free(SmModeller *m) {
+    if (m == nullptr) return;
+    for (int ty = 0; ty < 3; ++ty)
+        for (int k = 0; k < 2; ++k)
+            if (m->ev[ty][k] != nullptr) (void)hipEventDestroy(m->ev[ty][k]);
+    delete m;
+}
+
+extern "C" const char *sm_update_with_a_source(SmModeller *m, const SmComponent *comps, int32_t num_comps,
+                                               double pc_ra, double pc_dec) {
+    if (m == nullptr || (num_comps > 0 && comps == nullptr)) return "null argument";
+    m->pc_ra = pc_ra;
+    m->pc_dec = pc_dec;
+    std::vector<const SmComponent *> order;
+    for (int i = 0; i < num_comps; ++i) order.push_back(&comps[i]);
+    return set_components(m, order.data(), (int)order.size());
+}
+
+extern "C" const char *sm_model_timestep_with_device(SmModeller *m, double lst, double latitude, const UVW *d_uvws,
+                                                     JonesF32 *d_vis_fb, JonesF64 *d_vis_fb_f64) {
+    if (m == nullptr || d_vis_fb == nullptr) return "null argument";
+    if (m->nbl == 0) return nullptr;
+    const UVW *uvw = d_uvws;
+    if (uvw == nullptr) {
+        cross_uvws(m->xyz, m->nt, lst - m->pc_ra, m->pc_dec, m->h_uvw);
+        SM_TRY(upload(m->d_uvw, m->h_uvw));
+        uvw = m->d_uvw.as<UVW>();
+    }
+    ModelCtx ctx;
+    ctx.nbl = m->nbl;
+    ctx.nchan = m->nchan;
+    ctx.nchan_pad = m->nchan_pad;
+    ctx.ntiles = m->nt;
+    ctx.d_freqs_pad = m->d_freqs.as<double>();
+    ctx.even = m->even;
+    ctx.d_uvw = uvw;
+    ctx.d_sbf = m->d_sbf.as<double>();
+    const Kind kinds[3] = {KIND_POINT, KIND_GAUSSIAN, KIND_SHAPELET};
+    for (int ty = 0; ty < 3; ++ty) {
+        DevType &d = m->types[ty];
+        if (d.h.n() == 0) continue;
+        if (const char *e = prepare_beam(m, d, lst, latitude, ctx.beam)) return e;
+        if (ty == SM_SHAPELET)
+            if (const char *e = prepare_shapelet_uvs(m, d, lst)) return e;
+        TypeInputs in = type_inputs(d);
+        hipEvent_t e0 = m->timing ? m->ev[ty][0] : nullptr, e1 = m->timing ? m->ev[ty][1] : nullptr;
+        SM_TRY(run_type(kinds[ty], in, ctx, d.coh, d_vis_fb, d_vis_fb_f64, 0, e0, e1));
+        if (m->timing) {
+            SM_TRY(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            SM_TRY(hipEventElapsedTime(&ms, e0, e1));
+            m->ms_type[ty] += ms;
+            m->ms_total += ms;
+            m->launches++;
+        }
+    }
+    return nullptr;
+}
+
+extern "C" const char *sm_model_timestep_with(SmModeller *m, double lst, double latitude, UVW *uvws_out,
+                                              JonesF32 *vis_fb, JonesF64 *vis_fb_f64) {
+    if (m == nullptr || vis_fb == nullptr) return "null argument";
+    const size_t nvis = (size_t)m->nchan * m->nbl;
+    cross_uvws(m->xyz, m->nt, lst - m->pc_ra, m->pc_dec, m->h_uvw);
+    if (uvws_out != nullptr) std::memcpy(uvws_out, m->h_uvw.data(), sizeof(UVW) * m->h_uvw.size());
+    if (nvis == 0) return nullptr;
+    SM_TRY(m->d_vis32.reserve(sizeof(JonesF32) * nvis));
+    SM_TRY(hipMemcpy(m->d_vis32.ptr, vis_fb, sizeof(JonesF32) * nvis, hipMemcpyHostToDevice));
+    JonesF64 *d64 = nullptr;
+    if (vis_fb_f64 != nullptr) {
+        SM_TRY(m->d_vis64.reserve(sizeof(JonesF64) * nvis));
+        SM_TRY(hipMemcpy(m->d_vis64.ptr, vis_fb_f64, sizeof(JonesF64) * nvis, hipMemcpyHostToDevice));
+        d64 = m->d_vis64.as<JonesF64>();
+    }
+    if (const char *e = sm_model_timestep_with_device(m, lst, latitude, nullptr, m->d_vis32.as<JonesF32>(), d64))
+        return e;
+    SM_TRY(hipMemcpy(vis_fb, m->d_vis32.ptr, sizeof(JonesF32) * nvis, hipMemcpyDeviceToHost));
+    if (vis_fb_f64 != nullptr)
+        SM_TRY(hipMemcpy(vis_fb_f64, m->d_vis64.ptr, sizeof(JonesF64) * nvis, hipMemcpyDeviceToHost));
+    mask_pols(m->pols, vis_fb, vis_fb_f64, nvis);
+    return nullptr;
+}
+
+extern "C" const char *sm_model_timestep_autos_with(SmModeller *m, double lst, double latitude, JonesF32 *vis_fb,
+                                                    JonesF64 *vis_fb_f64) {
+    if (m == nullptr || vis_fb == nullptr) return "null argument";
+    const size_t nvis = (size_t)m->nchan * m->nt;
+    SM_TRY(m->d_vis32.reserve(sizeof(JonesF32) * nvis));
+    SM_TRY(hipMemcpy(m->d_vis32.ptr, vis_fb, sizeof(JonesF32) * nvis, hipMemcpyHostToDevice));
+    JonesF64 *d64 = nullptr;
+    if (vis_fb_f64 != nullptr) {
+        SM_TRY(m->d_vis64.reserve(sizeof(JonesF64) * nvis));
+        SM_TRY(hipMemcpy(m->d_vis64.ptr, vis_fb_f64, sizeof(JonesF64) * nvis, hipMemcpyHostToDevice));
+        d64 = m->d_vis64.as<JonesF64>();
+    }
+    // One demotion per component type, as SkyModellerCpu does (src/model/cpu.rs:906-932).
+    for (int ty = 0; ty < 3; ++ty) {
+        DevType &d = m->types[ty];
+        const int n = d.h.n();
+        if (n == 0) continue;
+        BeamDesc b;
+        if (const char *e = prepare_beam(m, d, lst, latitude, b)) return e;
+        SM_TRY(d.coh.reserve(sizeof(double) * 8 * (size_t)n * m->nchan_pad));
+        TypeInputs in = type_inputs(d);
+        BeamDesc none = b;
+        none.jones = nullptr;
+        SM_TRY(launch_coherency(n, m->nchan, m->nchan_pad, m->d_freqs.as<double>(), in.fb, none, false,
+                                d.coh.as<double>(), 0));
+        b.uniform_row = -1;
+        SM_TRY(launch_autos(m->nt, m->nchan, n, d.coh.as<double>(), m->nchan_pad, b, false, m->d_vis32.as<JonesF32>(),
+                            d64, nullptr, true, 0));
+    }
+    SM_TRY(hipMemcpy(vis_fb, m->d_vis32.ptr, sizeof(JonesF32) * nvis, hipMemcpyDeviceToHost));
+    if (vis_fb_f64 != nullptr)
+        SM_TRY(hipMemcpy(vis_fb_f64, m->d_vis64.ptr, sizeof(JonesF64) * nvis, hipMemcpyDeviceToHost));
+    mask_pols(m->pols, vis_fb, vis_fb_f64, nvis);
+    return nullptr;
+}
+
+extern "C" const char *sm_get_info(SmModeller *m, SmInfo *info) {
+    if (m == nullptr || info == nullptr) return "null argument";
+    info->num_baselines = m->nbl;
+    info->num_freqs = m->nchan;
+    info->num_unflagged_tiles = m->nt;
+    info->num_unique_tiles = m->n_utile;
+    info->num_points = m->types[0].h.n();
+    info->num_gaussians = m->types[1].h.n();
+    info->num_shapelets = m->types[2].h.n();
+    info->num_kernel_launches = m->launches;
+    info->kernel_ms = m->ms_total;
+    info->kernel_ms_points = m->ms_type[0];
+    info->kernel_ms_gaussians = m->ms_type[1];
+    info->kernel_ms_shapelets = m->ms_type[2];
+    return nullptr;
+}
+
+extern "C" const char *sm_set_timing(SmModeller *m, int32_t enabled) {
+    if (m == nullptr) return "null argument";
+    m->timing = enabled != 0;
+    m->launches = 0;
+    m->ms_total = 0;
+    m->ms_type[0] = m->ms_type[1] = m->ms_type[2] = 0;
+    return nullptr;
+}
+
+extern "C" const char *sm_synchronize(void) {
+    SM_TRY(hipDeviceSynchronize());
+    return nullptr;
+}

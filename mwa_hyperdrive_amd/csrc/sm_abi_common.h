@@ -1,0 +1,36 @@
+// Shared driver for one component type, used by the kernel-level drop-ins (abi.hip) and
+// the SkyModeller handle (modeller.hip).
+#pragma once
+
+#include "sm_internal.h"
+
+namespace sm {
+
+// Device-resident description of one component type.
+struct TypeInputs {
+    FluxBuckets fb;
+    const double4 *lmn = nullptr;
+    const double4 *shape = nullptr;
+    const double2 *bl_uv = nullptr;
+    const int2 *coeff_range = nullptr;
+    const ShapeletCoeff *coeffs = nullptr;
+};
+
+// Per-call modelling context.
+struct ModelCtx {
+    int nbl = 0, nchan = 0, nchan_pad = 0, ntiles = 0;
+    const double *d_freqs_pad = nullptr;
+    bool even = false;
+    const UVW *d_uvw = nullptr;
+    const double *d_sbf = nullptr;
+    BeamDesc beam{};
+};
+
+// Coherency table + direct kernel for one type; ev0/ev1 (nullable) bracket the direct kernel.
+hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf &coh, JonesF32 *vis32,
+                    JonesF64 *vis64, hipStream_t stream, hipEvent_t ev0, hipEvent_t ev1);
+
+double4 host_gaussian_shape(GaussianParams g);
+double4 host_shapelet_shape(GaussianParams g);
+
+}  // namespace sm

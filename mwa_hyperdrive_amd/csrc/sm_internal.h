@@ -1,0 +1,106 @@
+// Internal interfaces of the MI355X sky-model predictor (not part of the C ABI).
+//
+// Device data layout (per component type, components concatenated in the reference's
+// bucket order power law -> curved power law -> list, src/model/gpu.rs:889-910):
+//   lmn4  [n]                    double4 (2pi l, 2pi m, 2pi (n-1), 0)
+//   shape [n]                    double4: Gaussians (A, B, C, 0) of the envelope exponent
+//                                A u^2 + B u v + C v^2 (u, v in metres; includes the
+//                                -(pi/2)^2/ln2 factor and 1/c^2); shapelets
+//                                (sin pa, cos pa, maj*K/c, -min*K/c) with K = sqrt(pi^2/(2 ln2))/0.01
+//   coh   [n][nchan_pad][8]      f64: J1 S J2^H when every tile shares one beam (or there is
+//                                no beam), otherwise the instrumental flux S alone
+//   bl_uv [n][nbl]               double2 shapelet (u, v) metres, component as phase centre
+//   vis   [nchan][nbl]           JonesF32 (+=) and optionally JonesF64 (+=)
+// Channels are processed in chunks of SM_NCH; nchan_pad rounds nchan up to a whole chunk.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/skymodel.h"
+
+#define SM_NCH 8
+#define SM_VEL_C 299792458.0
+
+namespace sm {
+
+enum Kind { KIND_POINT = 0, KIND_GAUSSIAN = 1, KIND_SHAPELET = 2 };
+
+// Flux description of one component type as the reference's GPU structs hold it.
+struct FluxBuckets {
+    int n_pl, n_cpl, n_list;
+    const JonesF64 *pl_fds;
+    const double *pl_sis;
+    const JonesF64 *cpl_fds;
+    const double *cpl_sis;
+    const double *cpl_qs;
+    const JonesF64 *list_fds;  // [nchan][n_list]
+};
+
+// How the beam enters: none (identity), one shared beam row, or per-tile rows.
+struct BeamDesc {
+    const JonesF64 *jones;  // [n_utile][n_ufreq][n_dir] or nullptr (identity)
+    const int *freq_map;    // [nchan] -> unique freq
+    int n_ufreq;
+    int n_dir;              // directions per (tile, freq) row of `jones`
+    int dir_offset;         // first direction of this component type within a row
+    int uniform_row;        // unique-tile row used by every tile (uniform mode), else -1
+    const int2 *bl_rows;    // [nbl] unique-tile rows of (tile1, tile2) (per-tile mode)
+    const int *tile_rows;   // [ntiles] unique-tile row per unflagged tile (autos)
+};
+
+struct DirectArgs {
+    int nbl, nchan, nchan_pad, ncomp;
+    const double *freqs;      // [nchan_pad]
+    const UVW *uvw;           // [nbl]
+    const double4 *lmn;       // [ncomp]
+    const double4 *shape;     // [ncomp]
+    const double *coh;        // [ncomp][nchan_pad][8]
+    const double2 *bl_uv;     // [ncomp][nbl]
+    const int2 *coeff_range;  // [ncomp] (offset, count)
+    const ShapeletCoeff *coeffs;
+    const double *sbf;        // [SM_SBF_N][SM_SBF_L]
+    BeamDesc beam;            // per-tile mode only
+    JonesF32 *vis32;          // [nchan][nbl]
+    JonesF64 *vis64;          // nullable
+};
+
+// Launchers (direct.hip). All enqueue on `stream` and return hipSuccess or an error.
+hipError_t launch_coherency(int ncomp, int nchan, int nchan_pad, const double *d_freqs, FluxBuckets fb,
+                            BeamDesc beam, bool apply_beam, double *d_coh, hipStream_t stream);
+hipError_t launch_direct(Kind kind, bool per_tile, bool even, const DirectArgs &a, hipStream_t stream);
+bool chunks_even(const double *freqs_padded, int nchan);
+hipError_t launch_autos(int ntiles, int nchan, int ncomp, const double *d_coh, int nchan_pad, BeamDesc beam,
+                        bool coh_has_beam, JonesF32 *vis32, JonesF64 *vis64, double *accum64, bool demote_now,
+                        hipStream_t stream);
+hipError_t launch_autos_demote(int n, const double *accum64, JonesF32 *vis32, JonesF64 *vis64, hipStream_t stream);
+hipError_t launch_shapelet_bl_uv(int ncomp, int ntiles, int nbl, const double *d_comp_hadec, const double *d_xyz,
+                                 const int2 *d_bl_tiles, double2 *d_bl_uv, hipStream_t stream);
+hipError_t launch_repack_shapelet_uv(int nbl, int n_pl, int n_cpl, int n_list, const ShapeletUV *pl,
+                                     const ShapeletUV *cpl, const ShapeletUV *list, double2 *d_bl_uv,
+                                     hipStream_t stream);
+hipError_t launch_analytic_beam(int n_tiles, const unsigned *d_delays, const double *d_gains, int n_freqs,
+                                const double *d_freqs, int n_dirs, const double *d_az, const double *d_za,
+                                JonesF64 *d_out, hipStream_t stream);
+
+// Host helpers (abi.hip).
+int pad_channels(int nchan);
+// Pads a frequency list to whole channel chunks by continuing its last step.
+void pad_freqs(const double *freqs, int nchan, double *out_padded);
+
+// A grow-only device buffer.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    hipError_t reserve(size_t n);
+    void release();
+    ~DevBuf() { release(); }
+    template <class T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+}  // namespace sm
+
+#define SM_TRY(call)                                   \
+    do {                                               \
+        hipError_t sm_err_ = (call);                   \
+        if (sm_err_ != hipSuccess) return hipGetErrorString(sm_err_); \
+    } while (0)

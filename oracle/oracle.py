@@ -120,7 +120,7 @@ def model_timestep(obs: Obs, srclist, mask: int = 7, num_threads: int = 0, vis_f
     v32 = np.zeros(shape, np.float32) if vis_f32 is None else vis_f32
     v64 = np.zeros(shape, np.float64) if vis_f64 is None else vis_f64
     co = obs.c()
-    nt = num_threads or os.cpu_count() or 1
+    nt = num_threads or min(16, os.cpu_count() or 1)
     rc = L.orc_model_timestep(C.byref(co), C.cast(packed.comps, C.c_void_p), C.cast(packed.offsets, C.c_void_p),
                               packed.num_sources, mask, _ptr(v32), _ptr(v64), nt)
     if rc != 0:
@@ -135,7 +135,7 @@ def model_timestep_slice(obs: Obs, srclist, chan_start: int, chan_count: int, nu
     v32 = np.zeros(shape, np.float32)
     v64 = np.zeros(shape, np.float64)
     co = obs.c()
-    nt = num_threads or os.cpu_count() or 1
+    nt = num_threads or min(16, os.cpu_count() or 1)
     rc = L.orc_model_timestep_slice(C.byref(co), C.cast(packed.comps, C.c_void_p), C.cast(packed.offsets, C.c_void_p),
                                     packed.num_sources, chan_start, chan_count, _ptr(v32), _ptr(v64), nt)
     if rc != 0:

@@ -198,9 +198,14 @@ struct SmModeller {
     DevBuf d_freqs, d_xyz, d_bl_tiles, d_bl_rows, d_tile_rows, d_freq_map, d_ufreqs, d_delays, d_gains, d_sbf;
     DevBuf d_uvw, d_vis32, d_vis64, d_accum;
     DevType types[3];
-    // timing
+    // timing: event pairs bracketing each direct-kernel launch, resolved lazily
     bool timing = false;
-    hipEvent_t ev[3][2] = {};
+    struct Timed {
+        hipEvent_t e0, e1;
+        int type;
+    };
+    std::vector<Timed> pending;
+    std::vector<hipEvent_t> pool;
     int launches = 0;
     double ms_total = 0, ms_type[3] = {0, 0, 0};
     std::vector<UVW> h_uvw;
@@ -491,8 +496,6 @@ extern "C" const char *sm_new(const SmObsParams *p, const SmComponent *comps, co
     SM_NEW_TRY(upload(m->d_delays, m->delays));
     SM_NEW_TRY(upload(m->d_gains, m->gains));
     SM_NEW_TRY(upload(m->d_sbf, shapelet_table()));
-    for (int ty = 0; ty < 3; ++ty)
-        for (int k = 0; k < 2; ++k) SM_NEW_TRY(hipEventCreate(&m->ev[ty][k]));
 #undef SM_NEW_TRY
     // Reverse the source order (src/model/gpu.rs:230-241, src/model/cpu.rs:86-93).
     std::vector<const SmComponent *> order;
@@ -505,9 +508,12 @@ extern "C" const char *sm_new(const SmObsParams *p, const SmComponent *comps, co
 
 extern "C" void sm_free(SmModeller *m) {
     if (m == nullptr) return;
-    for (int ty = 0; ty < 3; ++ty)
-        for (int k = 0; k < 2; ++k)
-            if (m->ev[ty][k] != nullptr) (void)hipEventDestroy(m->ev[ty][k]);
+    (void)hipDeviceSynchronize();
+    for (auto &t : m->pending) {
+        (void)hipEventDestroy(t.e0);
+        (void)hipEventDestroy(t.e1);
+    }
+    for (hipEvent_t e : m->pool) (void)hipEventDestroy(e);
     delete m;
 }
 
@@ -548,16 +554,19 @@ extern "C" const char *sm_model_timestep_with_device(SmModeller *m, double lst, 
         if (ty == SM_SHAPELET)
             if (const char *e = prepare_shapelet_uvs(m, d, lst)) return e;
         TypeInputs in = type_inputs(d);
-        hipEvent_t e0 = m->timing ? m->ev[ty][0] : nullptr, e1 = m->timing ? m->ev[ty][1] : nullptr;
-        SM_TRY(run_type(kinds[ty], in, ctx, d.coh, d_vis_fb, d_vis_fb_f64, 0, e0, e1));
+        hipEvent_t e0 = nullptr, e1 = nullptr;
         if (m->timing) {
-            SM_TRY(hipEventSynchronize(e1));
-            float ms = 0.0f;
-            SM_TRY(hipEventElapsedTime(&ms, e0, e1));
-            m->ms_type[ty] += ms;
-            m->ms_total += ms;
-            m->launches++;
+            for (hipEvent_t *e : {&e0, &e1}) {
+                if (!m->pool.empty()) {
+                    *e = m->pool.back();
+                    m->pool.pop_back();
+                } else {
+                    SM_TRY(hipEventCreate(e));
+                }
+            }
+            m->pending.push_back(SmModeller::Timed{e0, e1, ty});
         }
+        SM_TRY(run_type(kinds[ty], in, ctx, d.coh, d_vis_fb, d_vis_fb_f64, 0, e0, e1));
     }
     return nullptr;
 }
@@ -622,8 +631,25 @@ extern "C" const char *sm_model_timestep_autos_with(SmModeller *m, double lst, d
     return nullptr;
 }
 
+// Folds the completed event pairs into the per-type totals.
+static const char *resolve_timing(SmModeller *m) {
+    for (auto &t : m->pending) {
+        SM_TRY(hipEventSynchronize(t.e1));
+        float ms = 0.0f;
+        SM_TRY(hipEventElapsedTime(&ms, t.e0, t.e1));
+        m->ms_type[t.type] += ms;
+        m->ms_total += ms;
+        m->launches++;
+        m->pool.push_back(t.e0);
+        m->pool.push_back(t.e1);
+    }
+    m->pending.clear();
+    return nullptr;
+}
+
 extern "C" const char *sm_get_info(SmModeller *m, SmInfo *info) {
     if (m == nullptr || info == nullptr) return "null argument";
+    if (const char *e = resolve_timing(m)) return e;
     info->num_baselines = m->nbl;
     info->num_freqs = m->nchan;
     info->num_unflagged_tiles = m->nt;
@@ -641,6 +667,7 @@ extern "C" const char *sm_get_info(SmModeller *m, SmInfo *info) {
 
 extern "C" const char *sm_set_timing(SmModeller *m, int32_t enabled) {
     if (m == nullptr) return "null argument";
+    if (const char *e = resolve_timing(m)) return e;
     m->timing = enabled != 0;
     m->launches = 0;
     m->ms_total = 0;

@@ -134,6 +134,11 @@ const char *sm_get_info(SmModeller *m, SmInfo *info);
 /* Enables (1) or disables (0) per-kernel HIP-event timing and resets the counters. */
 const char *sm_set_timing(SmModeller *m, int32_t enabled);
 const char *sm_synchronize(void);
+/* Kernel choice for the direct path: 0 (default) picks the fastest kernel for the case
+ * (tile-pair phasor kernel for points and Gaussians with evenly spaced channels and one
+ * shared beam, channel-per-lane kernel for shapelets); 1 forces the baseline-per-lane
+ * kernels everywhere. Process-wide; used to compare the two. */
+const char *sm_set_direct_variant(int32_t variant);
 
 /* ------------------------------------------------------------------ beam */
 /* Analytic MWA-like beam on the device: out[tile][freq][dir] JonesF64 for the given unique

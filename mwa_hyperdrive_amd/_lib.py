@@ -139,6 +139,7 @@ SIGNATURES = {
     "sm_get_info": [_P, C.POINTER(SmInfo)],
     "sm_set_timing": [_P, _I],
     "sm_synchronize": [],
+    "sm_set_direct_variant": [_I],
     "sm_analytic_beam": [_I, _P, _P, _I, _P, _I, _P, _P, _P],
     "sm_model_points_factored": [_P, _D, _D, _I, _P, _P],
     "sm_factored_info": [_P, _P],

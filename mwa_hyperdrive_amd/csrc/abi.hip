@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "sm_abi_common.h"
+#include "sm_tiled.h"
 
 namespace sm {
 
@@ -105,17 +106,61 @@ hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf
     a.vis32 = vis32;
     a.vis64 = vis64;
     if (ev0 != nullptr) (void)hipEventRecord(ev0, stream);
-    e = launch_direct(kind, per_tile, ctx.even, a, stream);
+    if (g_direct_variant == 0 && kind != KIND_SHAPELET && !per_tile && ctx.even && ctx.d_tile_uvw != nullptr &&
+        ctx.d_pairs != nullptr && ctx.ntiles >= 2) {
+        TiledArgs t;
+        t.ntiles = ctx.ntiles;
+        t.nbl = ctx.nbl;
+        t.nchan = ctx.nchan;
+        t.nchan_pad = ctx.nchan_pad;
+        t.ncomp = n;
+        t.npairs = ctx.npairs;
+        t.nchunks = (ctx.nchan + SM_NCH - 1) / SM_NCH;
+        t.pairs = ctx.d_pairs;
+        t.freqs = ctx.d_freqs_pad;
+        t.tile_uvw = ctx.d_tile_uvw;
+        t.lmn = in.lmn;
+        t.shape = in.shape;
+        t.coh = coh.as<double>();
+        t.vis32 = vis32;
+        t.vis64 = vis64;
+        e = launch_direct_tiled(kind, t, stream);
+    } else if (g_direct_variant == 0 && kind == KIND_SHAPELET) {
+        ShapeletChanArgs s;
+        s.nbl = ctx.nbl;
+        s.nchan = ctx.nchan;
+        s.nchan_pad = ctx.nchan_pad;
+        s.ncomp = n;
+        s.ngroups = (ctx.nchan + 63) / 64;
+        s.freqs = ctx.d_freqs_pad;
+        s.uvw = ctx.d_uvw;
+        s.lmn = in.lmn;
+        s.shape = in.shape;
+        s.coh = coh.as<double>();
+        s.bl_uv = in.bl_uv;
+        s.coeff_range = in.coeff_range;
+        s.coeffs = in.coeffs;
+        s.sbf = ctx.d_sbf;
+        s.beam = ctx.beam;
+        s.vis32 = vis32;
+        s.vis64 = vis64;
+        e = launch_shapelet_chan(per_tile, s, stream);
+    } else {
+        e = launch_direct(kind, per_tile, ctx.even, a, stream);
+    }
     if (ev1 != nullptr) (void)hipEventRecord(ev1, stream);
     return e;
 }
+
+int g_direct_variant = 0;
 
 // ------------------------------------------------------------------ ABI context
 namespace {
 
 struct AbiState {
-    DevBuf freqs_pad, lmn, shape, coh, bl_rows, bl_uv, coeffs, coeff_range, sbf, accum, tile_rows;
+    DevBuf freqs_pad, lmn, shape, coh, bl_rows, bl_uv, coeffs, coeff_range, sbf, accum, tile_rows, tile_uvw, pairs;
     std::vector<double> h_freqs_pad;
+    int pairs_ntiles = -1, npairs = 0;
 };
 AbiState &abi_state() {
     static AbiState s;
@@ -144,6 +189,20 @@ const char *abi_ctx(const Addresses *a, const UVW *d_uvws, const JonesF64 *d_bea
     ctx.d_uvw = d_uvws;
     ctx.d_sbf = a->d_shapelet_basis_values;
     ctx.ntiles = ntiles;
+    if (d_uvws != nullptr && ntiles >= 2) {
+        SM_TRY(st.tile_uvw.reserve(sizeof(double) * 3 * ntiles));
+        SM_TRY(launch_tile_uvw_from_baselines(ntiles, d_uvws, st.tile_uvw.as<double>(), 0));
+        ctx.d_tile_uvw = st.tile_uvw.as<double>();
+        if (st.pairs_ntiles != ntiles) {
+            const std::vector<int4> pairs = tile_pairs(ntiles);
+            SM_TRY(st.pairs.reserve(sizeof(int4) * pairs.size()));
+            SM_TRY(hipMemcpy(st.pairs.ptr, pairs.data(), sizeof(int4) * pairs.size(), hipMemcpyHostToDevice));
+            st.pairs_ntiles = ntiles;
+            st.npairs = (int)pairs.size();
+        }
+        ctx.d_pairs = st.pairs.as<int4>();
+        ctx.npairs = st.npairs;
+    }
     BeamDesc &b = ctx.beam;
     b.jones = d_beam;
     b.freq_map = a->d_freq_map;
@@ -430,5 +489,11 @@ extern "C" const char *sm_analytic_beam(int32_t n_tiles, const uint32_t *delays,
     SM_TRY(launch_analytic_beam(n_tiles, dd.as<unsigned>(), dg.as<double>(), n_freqs, df.as<double>(), n_dirs, d_az,
                                 d_za, d_out, 0));
     SM_TRY(hipStreamSynchronize(0));
+    return nullptr;
+}
+
+extern "C" const char *sm_set_direct_variant(int32_t variant) {
+    if (variant < 0 || variant > 1) return "variant must be 0 (fastest) or 1 (baseline per lane)";
+    sm::g_direct_variant = variant;
     return nullptr;
 }

@@ -1,0 +1,317 @@
+// Faster FP64 direct kernels for the common case: one beam shared by every tile (or no
+// beam) and evenly spaced channels.
+//
+// k_direct_tiled (points, Gaussians): a block owns a 16 x 16 block of tile pairs and one
+// chunk of SM_NCH channels. The baseline phase factorises over antennas,
+//   e^{i theta_pq f/c} = e^{i theta_p f/c} e^{-i theta_q f/c},  theta_t = u_t L + v_t M + w_t N,
+// (baseline uvw = tile1 - tile2, mdbook/src/defs/coords.md), so per component the block
+// evaluates the start phasor and the per-channel step for its 32 tiles into LDS (64
+// sincos shared by 256 baselines instead of 512), and each lane forms its baseline's
+// phasors with two complex products. Diagonal tile blocks are paired so a block keeps 240
+// of its 256 lanes busy.
+//
+// k_shapelet_chan (shapelets): one wave per baseline, one lane per channel. The basis-table
+// positions x_pos = k f + 5000 of neighbouring channels fall in the same or adjacent table
+// cells, so each coefficient's table reads coalesce across the wave instead of touching
+// 64 unrelated cache lines.
+#include "sm_internal.h"
+#include "sm_math.h"
+#include "sm_tiled.h"
+
+namespace sm {
+
+constexpr int kTG = 16;   // tiles per group
+constexpr int kCT = 32;   // components per LDS stage
+
+template <int KIND>
+__global__ __launch_bounds__(256) void k_direct_tiled(const TiledArgs a) {
+    __shared__ double4 s_ph[kCT][32];
+    __shared__ double s_uvw[32][3];
+    const int total = a.npairs * a.nchunks;
+    int b = blockIdx.x;
+    if ((total & 7) == 0) b = (b & 7) * (total >> 3) + (b >> 3);
+    const int chunk = b / a.npairs;
+    const int4 pr = a.pairs[b - chunk * a.npairs];
+    const int tid = threadIdx.x, r = tid >> 4, cq = tid & 15;
+    if (tid < 32) {
+        const int g = tid < 16 ? pr.y : pr.z;
+        int t = g * kTG + (tid & 15);
+        if (g < 0 || t >= a.ntiles) t = 0;
+        s_uvw[tid][0] = a.tile_uvw[3 * t];
+        s_uvw[tid][1] = a.tile_uvw[3 * t + 1];
+        s_uvw[tid][2] = a.tile_uvw[3 * t + 2];
+    }
+    int p, q, sp, sq;
+    bool valid;
+    if (pr.x == 0) {
+        p = pr.y * kTG + r;
+        q = pr.z * kTG + cq;
+        sp = r;
+        sq = 16 + cq;
+        valid = q < a.ntiles;
+    } else if (r < cq) {
+        p = pr.y * kTG + r;
+        q = pr.y * kTG + cq;
+        sp = r;
+        sq = cq;
+        valid = q < a.ntiles;
+    } else if (r > cq && pr.z >= 0) {
+        p = pr.z * kTG + cq;
+        q = pr.z * kTG + r;
+        sp = 16 + cq;
+        sq = 16 + r;
+        valid = q < a.ntiles;
+    } else {
+        p = 0;
+        q = 1;
+        sp = 0;
+        sq = 1;
+        valid = false;
+    }
+    __syncthreads();
+    const double bu = s_uvw[sp][0] - s_uvw[sq][0];
+    const double bv = s_uvw[sp][1] - s_uvw[sq][1];
+
+    const int f0 = chunk * SM_NCH;
+    const double *__restrict__ freqs = a.freqs;
+    const double F0 = freqs[f0];
+    const double DF = freqs[f0 + 1] - F0;
+    const double f0c = F0 / SM_VEL_C;
+    const double dfc = DF / SM_VEL_C;
+    const double *__restrict__ coh = a.coh;
+    const double4 *__restrict__ lmn4 = a.lmn;
+    const double4 *__restrict__ shape = a.shape;
+
+    double acc[SM_NCH][8];
+#pragma unroll
+    for (int k = 0; k < SM_NCH; ++k)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) acc[k][e] = 0.0;
+
+    for (int c0 = 0; c0 < a.ncomp; c0 += kCT) {
+        const int nc = a.ncomp - c0 < kCT ? a.ncomp - c0 : kCT;
+        // Per-tile phasors (start, step) of this stage's components.
+        for (int e = tid; e < kCT * 32; e += 256) {
+            const int slot = e & 31, cc = e >> 5;
+            double4 ph = make_double4(1.0, 0.0, 1.0, 0.0);
+            if (cc < nc) {
+                const double4 l = lmn4[c0 + cc];
+                const double th = s_uvw[slot][0] * l.x + s_uvw[slot][1] * l.y + s_uvw[slot][2] * l.z;
+                fast_sincos(th * f0c, ph.y, ph.x);
+                fast_sincos(th * dfc, ph.w, ph.z);
+            }
+            s_ph[cc][slot] = ph;
+        }
+        __syncthreads();
+        for (int cc = 0; cc < nc; ++cc) {
+            const int c = c0 + cc;
+            const double4 A = s_ph[cc][sp];
+            const double4 B = s_ph[cc][sq];
+            double zr = A.x * B.x + A.y * B.y;
+            double zi = A.y * B.x - A.x * B.y;
+            const double rr = A.z * B.z + A.w * B.w;
+            const double ri = A.w * B.z - A.z * B.w;
+            double env = 1.0, er = 1.0, eg = 1.0;
+            if (KIND == KIND_GAUSSIAN) {
+                const double4 g = shape[c];
+                const double qf = g.x * (bu * bu) + g.y * (bu * bv) + g.z * (bv * bv);
+                env = exp(qf * (F0 * F0));
+                er = exp(qf * (DF * (2.0 * F0 + DF)));
+                eg = exp(qf * (2.0 * DF * DF));
+            }
+            const double *cc8 = coh + ((size_t)c * a.nchan_pad + f0) * 8;
+#pragma unroll
+            for (int k = 0; k < SM_NCH; ++k) {
+                double pr_ = zr, pi_ = zi;
+                if (KIND == KIND_GAUSSIAN) {
+                    pr_ *= env;
+                    pi_ *= env;
+                }
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const double cr = cc8[k * 8 + 2 * e], ci = cc8[k * 8 + 2 * e + 1];
+                    acc[k][2 * e] = fma(cr, pr_, acc[k][2 * e]);
+                    acc[k][2 * e] = fma(-ci, pi_, acc[k][2 * e]);
+                    acc[k][2 * e + 1] = fma(cr, pi_, acc[k][2 * e + 1]);
+                    acc[k][2 * e + 1] = fma(ci, pr_, acc[k][2 * e + 1]);
+                }
+                const double nr = zr * rr - zi * ri;
+                zi = zr * ri + zi * rr;
+                zr = nr;
+                if (KIND == KIND_GAUSSIAN) {
+                    env *= er;
+                    er *= eg;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!valid) return;
+    const size_t bl = (size_t)p * a.ntiles - (size_t)p * (p + 1) / 2 + (size_t)(q - p - 1);
+#pragma unroll
+    for (int k = 0; k < SM_NCH; ++k) {
+        const int f = f0 + k;
+        if (f >= a.nchan) break;
+        const size_t o = (size_t)f * a.nbl + bl;
+        float *v32 = reinterpret_cast<float *>(&a.vis32[o]);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v32[e] += (float)acc[k][e];
+        if (a.vis64 != nullptr) {
+            double *v64 = reinterpret_cast<double *>(&a.vis64[o]);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) v64[e] += acc[k][e];
+        }
+    }
+}
+
+// Tile-group pairs: off-diagonal (0, g0, g1) with g0 < g1, then diagonal groups paired from
+// both ends (1, g, G-1-g), the middle one alone (1, g, -1) when G is odd.
+std::vector<int4> tile_pairs(int ntiles) {
+    const int G = (ntiles + kTG - 1) / kTG;
+    std::vector<int4> out;
+    for (int g0 = 0; g0 < G; ++g0)
+        for (int g1 = g0 + 1; g1 < G; ++g1) out.push_back(make_int4(0, g0, g1, 0));
+    for (int g = 0; g < G / 2; ++g) out.push_back(make_int4(1, g, G - 1 - g, 0));
+    if (G % 2 == 1) out.push_back(make_int4(1, G / 2, -1, 0));
+    return out;
+}
+
+hipError_t launch_direct_tiled(Kind kind, const TiledArgs &a, hipStream_t stream) {
+    if (a.ncomp <= 0 || a.nbl <= 0 || a.nchan <= 0) return hipSuccess;
+    const int total = a.npairs * a.nchunks;
+    if (kind == KIND_GAUSSIAN)
+        hipLaunchKernelGGL(k_direct_tiled<KIND_GAUSSIAN>, dim3(total), dim3(256), 0, stream, a);
+    else
+        hipLaunchKernelGGL(k_direct_tiled<KIND_POINT>, dim3(total), dim3(256), 0, stream, a);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ shapelets
+template <bool PER_TILE>
+__global__ __launch_bounds__(256) void k_shapelet_chan(const ShapeletChanArgs a) {
+    const int nbx = (a.nbl + 3) / 4;
+    const int total = nbx * a.ngroups;
+    int b = blockIdx.x;
+    if ((total & 7) == 0) b = (b & 7) * (total >> 3) + (b >> 3);
+    const int group = b / nbx;
+    const int bx = b - group * nbx;
+    const int lane = threadIdx.x & 63;
+    const int bl = bx * 4 + (int)(threadIdx.x >> 6);
+    if (bl >= a.nbl) return;  // whole wave; the kernel has no barriers
+    const int f_raw = group * 64 + lane;
+    const bool valid = f_raw < a.nchan;
+    const int f = valid ? f_raw : a.nchan - 1;
+    const double freq = a.freqs[f];
+    const double fc = freq / SM_VEL_C;
+    const UVW uvw = a.uvw[bl];
+    int2 rows = make_int2(0, 0);
+    int uf = 0;
+    if (PER_TILE) {
+        rows = a.beam.bl_rows[bl];
+        uf = a.beam.freq_map[f];
+    }
+    const ShapeletCoeff *__restrict__ coeffs = a.coeffs;
+    const double *__restrict__ sbf = a.sbf;
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int c = 0; c < a.ncomp; ++c) {
+        const double4 l = a.lmn[c];
+        const double theta = uvw.u * l.x + uvw.v * l.y + uvw.w * l.z;
+        double zr, zi;
+        fast_sincos(theta * fc, zi, zr);
+        const double4 sh = a.shape[c];
+        const double2 suv = a.bl_uv[(size_t)c * a.nbl + bl];
+        const double xk = (suv.x * sh.x + suv.y * sh.y) * sh.z;
+        const double yk = (suv.x * sh.y - suv.y * sh.x) * sh.w;
+        const double x_pos = xk * freq + SM_SBF_C;
+        const double y_pos = yk * freq + SM_SBF_C;
+        const double fx = floor(x_pos), fy = floor(y_pos);
+        const int ix = (int)fmin(fmax(fx, 0.0), (double)(SM_SBF_L - 2));
+        const int iy = (int)fmin(fmax(fy, 0.0), (double)(SM_SBF_L - 2));
+        const double wx = x_pos - fx, wy = y_pos - fy;
+        double er = 0.0, ei = 0.0;
+        const int2 range = a.coeff_range[c];
+        for (int j = 0; j < range.y; ++j) {
+            const ShapeletCoeff cf = coeffs[range.x + j];
+            const int n1 = cf.n1 < SM_SBF_N ? cf.n1 : SM_SBF_N - 1;
+            const int n2 = cf.n2 < SM_SBF_N ? cf.n2 : SM_SBF_N - 1;
+            const double *r1 = sbf + (size_t)n1 * SM_SBF_L + ix;
+            const double *r2 = sbf + (size_t)n2 * SM_SBF_L + iy;
+            const double x_low = r1[0], x_high = r1[1];
+            const double y_low = r2[0], y_high = r2[1];
+            const double u_value = x_low + (x_high - x_low) * wx;
+            const double v_value = y_low + (y_high - y_low) * wy;
+            const double t = (cf.value * u_value) * v_value;
+            switch ((cf.n1 + cf.n2) & 3) {
+            case 0: er += t; break;
+            case 1: ei += t; break;
+            case 2: er -= t; break;
+            default: ei -= t; break;
+            }
+        }
+        const double pr = zr * er - zi * ei;
+        const double pi = zr * ei + zi * er;
+        double cj[8];
+        const double *s = a.coh + ((size_t)c * a.nchan_pad + f) * 8;
+        if (PER_TILE) {
+            double sv[8], j1[8], j2[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) sv[e] = s[e];
+            load_jones_t(&a.beam.jones[((size_t)rows.x * a.beam.n_ufreq + uf) * a.beam.n_dir + a.beam.dir_offset + c], j1);
+            load_jones_t(&a.beam.jones[((size_t)rows.y * a.beam.n_ufreq + uf) * a.beam.n_dir + a.beam.dir_offset + c], j2);
+            sandwich_t(j1, sv, j2, cj);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) cj[e] = s[e];
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            acc[2 * e] = fma(cj[2 * e], pr, acc[2 * e]);
+            acc[2 * e] = fma(-cj[2 * e + 1], pi, acc[2 * e]);
+            acc[2 * e + 1] = fma(cj[2 * e], pi, acc[2 * e + 1]);
+            acc[2 * e + 1] = fma(cj[2 * e + 1], pr, acc[2 * e + 1]);
+        }
+    }
+    if (!valid) return;
+    const size_t o = (size_t)f * a.nbl + bl;
+    float *v32 = reinterpret_cast<float *>(&a.vis32[o]);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v32[e] += (float)acc[e];
+    if (a.vis64 != nullptr) {
+        double *v64 = reinterpret_cast<double *>(&a.vis64[o]);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v64[e] += acc[e];
+    }
+}
+
+hipError_t launch_shapelet_chan(bool per_tile, const ShapeletChanArgs &a, hipStream_t stream) {
+    if (a.ncomp <= 0 || a.nbl <= 0 || a.nchan <= 0) return hipSuccess;
+    const int total = ((a.nbl + 3) / 4) * a.ngroups;
+    if (per_tile)
+        hipLaunchKernelGGL(k_shapelet_chan<true>, dim3(total), dim3(256), 0, stream, a);
+    else
+        hipLaunchKernelGGL(k_shapelet_chan<false>, dim3(total), dim3(256), 0, stream, a);
+    return hipGetLastError();
+}
+
+// Tile UVWs from baseline UVWs: r_0 = 0, r_q = -uvw(0, q), so r_p - r_q = uvw(p, q) for
+// UVWs that are differences of tile UVWs, as xyzs_to_cross_uvws produces.
+__global__ void k_tile_uvw_from_baselines(int ntiles, const UVW *__restrict__ uvw, double *__restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    if (t == 0) {
+        out[0] = out[1] = out[2] = 0.0;
+        return;
+    }
+    const UVW b = uvw[t - 1];  // baseline (0, t)
+    out[3 * t] = -b.u;
+    out[3 * t + 1] = -b.v;
+    out[3 * t + 2] = -b.w;
+}
+
+hipError_t launch_tile_uvw_from_baselines(int ntiles, const UVW *d_uvw, double *d_out, hipStream_t stream) {
+    hipLaunchKernelGGL(k_tile_uvw_from_baselines, dim3((ntiles + 255) / 256), dim3(256), 0, stream, ntiles, d_uvw,
+                       d_out);
+    return hipGetLastError();
+}
+
+}  // namespace sm

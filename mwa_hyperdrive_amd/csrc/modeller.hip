@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "sm_abi_common.h"
+#include "sm_tiled.h"
 
 namespace sm {
 namespace {
@@ -98,7 +99,8 @@ double4 lmn_rime(double ra, double dec, double pc_ra, double pc_dec) {
 }
 
 // Baseline UVWs, antenna1 - antenna2 over the upper triangle (mdbook/src/defs/coords.md).
-void cross_uvws(const std::vector<double> &xyz, int nt, double ha, double dec, std::vector<UVW> &out) {
+void cross_uvws(const std::vector<double> &xyz, int nt, double ha, double dec, std::vector<UVW> &out,
+                std::vector<double> *tile_out = nullptr) {
     const double s_ha = std::sin(ha), c_ha = std::cos(ha), s_dec = std::sin(dec), c_dec = std::cos(dec);
     std::vector<UVW> tile(nt);
     for (int i = 0; i < nt; ++i) {
@@ -106,6 +108,14 @@ void cross_uvws(const std::vector<double> &xyz, int nt, double ha, double dec, s
         tile[i].u = s_ha * x + c_ha * y;
         tile[i].v = -s_dec * c_ha * x + s_dec * s_ha * y + c_dec * z;
         tile[i].w = c_dec * c_ha * x - c_dec * s_ha * y + s_dec * z;
+    }
+    if (tile_out != nullptr) {
+        tile_out->resize(3 * (size_t)nt);
+        for (int i = 0; i < nt; ++i) {
+            (*tile_out)[3 * i] = tile[i].u;
+            (*tile_out)[3 * i + 1] = tile[i].v;
+            (*tile_out)[3 * i + 2] = tile[i].w;
+        }
     }
     out.resize((size_t)nt * (nt - 1) / 2);
     size_t b = 0;
@@ -196,7 +206,9 @@ struct SmModeller {
     bool even = false;
     // device
     DevBuf d_freqs, d_xyz, d_bl_tiles, d_bl_rows, d_tile_rows, d_freq_map, d_ufreqs, d_delays, d_gains, d_sbf;
-    DevBuf d_uvw, d_vis32, d_vis64, d_accum;
+    DevBuf d_uvw, d_vis32, d_vis64, d_accum, d_tile_uvw, d_pairs;
+    int npairs = 0;
+    std::vector<double> h_tile_uvw;
     DevType types[3];
     // timing: event pairs bracketing each direct-kernel launch, resolved lazily
     bool timing = false;
@@ -496,6 +508,11 @@ extern "C" const char *sm_new(const SmObsParams *p, const SmComponent *comps, co
     SM_NEW_TRY(upload(m->d_delays, m->delays));
     SM_NEW_TRY(upload(m->d_gains, m->gains));
     SM_NEW_TRY(upload(m->d_sbf, shapelet_table()));
+    {
+        const std::vector<int4> pairs = tile_pairs(m->nt);
+        m->npairs = (int)pairs.size();
+        SM_NEW_TRY(upload(m->d_pairs, pairs));
+    }
 #undef SM_NEW_TRY
     // Reverse the source order (src/model/gpu.rs:230-241, src/model/cpu.rs:86-93).
     std::vector<const SmComponent *> order;
@@ -532,10 +549,17 @@ extern "C" const char *sm_model_timestep_with_device(SmModeller *m, double lst, 
     if (m == nullptr || d_vis_fb == nullptr) return "null argument";
     if (m->nbl == 0) return nullptr;
     const UVW *uvw = d_uvws;
+    const double *tile_uvw = nullptr;
     if (uvw == nullptr) {
-        cross_uvws(m->xyz, m->nt, lst - m->pc_ra, m->pc_dec, m->h_uvw);
+        cross_uvws(m->xyz, m->nt, lst - m->pc_ra, m->pc_dec, m->h_uvw, &m->h_tile_uvw);
         SM_TRY(upload(m->d_uvw, m->h_uvw));
+        SM_TRY(upload(m->d_tile_uvw, m->h_tile_uvw));
         uvw = m->d_uvw.as<UVW>();
+        tile_uvw = m->d_tile_uvw.as<double>();
+    } else if (m->nt >= 2) {
+        SM_TRY(m->d_tile_uvw.reserve(sizeof(double) * 3 * m->nt));
+        SM_TRY(launch_tile_uvw_from_baselines(m->nt, uvw, m->d_tile_uvw.as<double>(), 0));
+        tile_uvw = m->d_tile_uvw.as<double>();
     }
     ModelCtx ctx;
     ctx.nbl = m->nbl;
@@ -546,6 +570,9 @@ extern "C" const char *sm_model_timestep_with_device(SmModeller *m, double lst, 
     ctx.even = m->even;
     ctx.d_uvw = uvw;
     ctx.d_sbf = m->d_sbf.as<double>();
+    ctx.d_tile_uvw = tile_uvw;
+    ctx.d_pairs = m->d_pairs.as<int4>();
+    ctx.npairs = m->npairs;
     const Kind kinds[3] = {KIND_POINT, KIND_GAUSSIAN, KIND_SHAPELET};
     for (int ty = 0; ty < 3; ++ty) {
         DevType &d = m->types[ty];

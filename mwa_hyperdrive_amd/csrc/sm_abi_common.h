@@ -24,7 +24,13 @@ struct ModelCtx {
     const UVW *d_uvw = nullptr;
     const double *d_sbf = nullptr;
     BeamDesc beam{};
+    const double *d_tile_uvw = nullptr;  // [ntiles][3], enables the tile-pair kernel
+    const int4 *d_pairs = nullptr;
+    int npairs = 0;
 };
+
+// 0: pick the fastest kernel for the case; 1: always the baseline-per-lane kernels.
+extern int g_direct_variant;
 
 // Coherency table + direct kernel for one type; ev0/ev1 (nullable) bracket the direct kernel.
 hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf &coh, JonesF32 *vis32,

@@ -32,6 +32,13 @@ pytestmark = pytest.mark.gpu
 dev = "cuda"
 
 
+@pytest.fixture(params=[0, 1], ids=["fast-kernels", "baseline-per-lane"])
+def variant(request):
+    _lib.call("sm_set_direct_variant", request.param)
+    yield request.param
+    _lib.call("sm_set_direct_variant", 0)
+
+
 def dt(a, dtype=torch.float64):
     return torch.tensor(np.ascontiguousarray(a), dtype=dtype, device=dev)
 
@@ -139,7 +146,7 @@ def addresses(freqs, nbl, n_ufreq, tile_map, freq_map, unflagged, sbf=None):
 
 @pytest.mark.parametrize("kind", ["point", "gaussian", "shapelet"])
 @pytest.mark.parametrize("beam", ["identity", "per_tile"])
-def test_dropin_vs_oracle(kind, beam):
+def test_dropin_vs_oracle(kind, beam, variant):
     from oracle import oracle as orc
     rng = np.random.default_rng(11)
     nt = 12
@@ -207,7 +214,7 @@ def test_dropin_vs_oracle(kind, beam):
     # The oracle reverses the source order; a single source keeps the bucket order.
     o32, o64 = orc.model_timestep(obs, sl, mask={"point": 1, "gaussian": 2, "shapelet": 4}[kind])
     err = rel_err(vis64.cpu().numpy(), o64)
-    print(f"{kind} {beam}: rel err {err:.3e}")
+    print(f"variant={variant} {kind} {beam}: rel err {err:.3e}")
     assert err <= 1e-12
 
 

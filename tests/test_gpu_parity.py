@@ -45,12 +45,22 @@ pytestmark = pytest.mark.gpu
 TOL_F64 = 1e-12  # relative to max|V|
 
 
+@pytest.fixture(params=[0, 1], ids=["fast-kernels", "baseline-per-lane"])
+def variant(request):
+    """Runs a test with the fastest kernels (tile-pair phasors, channel-per-lane shapelets)
+    and again with the baseline-per-lane kernels."""
+    from mwa_hyperdrive_amd import _lib
+    _lib.call("sm_set_direct_variant", request.param)
+    yield request.param
+    _lib.call("sm_set_direct_variant", 0)
+
+
 def fixture_modeller(srclist, pols=0):
     return SkyModeller(NoBeam(), srclist, pols, XYZS, FREQS, [], PHASE_CENTRE, 0.0, MWA_LAT_RAD)
 
 
 @pytest.mark.parametrize("checker,builder,pos,flux", NO_BEAM_CASES)
-def test_goldens_bit_exact(checker, builder, pos, flux):
+def test_goldens_bit_exact(checker, builder, pos, flux, variant):
     m = fixture_modeller(single(builder, pos, flux))
     vis, _ = m.model_timestep(0.0, MWA_LAT_RAD)
     np.testing.assert_array_equal(vis, golden(checker))
@@ -97,7 +107,7 @@ def _mixed_obs(beam, n_tiles=24, chans=None, dead=False, seed=7, comps=60):
 
 
 @pytest.mark.parametrize("beam,dead", [("none", False), ("analytic", False), ("analytic", True)])
-def test_mixed_sky_vs_oracle(beam, dead):
+def test_mixed_sky_vs_oracle(beam, dead, variant):
     """Points, Gaussians and shapelets with power-law, curved and list fluxes; identity,
     shared analytic and per-tile (dead-dipole) analytic beams; 37 channels so the last chunk
     is ragged."""
@@ -106,14 +116,14 @@ def test_mixed_sky_vs_oracle(beam, dead):
         v32, v64, _, m = gpu_model(obs, t)
         o32, o64 = oracle_model(obs, t)
         err = rel_err(v64, o64)
-        print(f"beam={beam} dead={dead} t={t} rel err {err:.3e} unique tiles {m.info()['num_unique_tiles']}")
+        print(f"variant={variant} beam={beam} dead={dead} t={t} rel err {err:.3e} unique tiles {m.info()['num_unique_tiles']}")
         assert err <= TOL_F64
         assert f32_ulps(v32, o32) <= 2.0 or rel_err(v32, o32) < 1e-7
     if dead:
         assert m.info()["num_unique_tiles"] > 1
 
 
-def test_uneven_channels_vs_oracle():
+def test_uneven_channels_vs_oracle(variant):
     """Channels that are not evenly spaced take the per-channel sincos/exp kernel."""
     freqs = 167e6 + 40e3 * np.array([0, 1, 2, 4, 5, 9, 10, 11, 12, 13, 20, 21], dtype=np.float64)
     obs = _mixed_obs("analytic", chans=freqs)
@@ -122,18 +132,18 @@ def test_uneven_channels_vs_oracle():
     assert rel_err(v64, o64) <= TOL_F64
 
 
-def test_c1_full_size_vs_oracle():
+def test_c1_full_size_vs_oracle(variant):
     """Configuration C1 in full: 128 tiles, 32 channels, 100 power-law points, no beam."""
     obs = synthetic.config(1)
     v32, v64, uvws, _ = gpu_model(obs, 0)
     o32, o64 = oracle_model(obs, 0)
     err = rel_err(v64, o64)
-    print(f"C1 rel err {err:.3e}")
+    print(f"variant={variant} C1 rel err {err:.3e}")
     assert err <= TOL_F64
     assert uvws.shape == (8128, 3)
 
 
-def test_c2_full_size_channel_slice_vs_oracle():
+def test_c2_full_size_channel_slice_vs_oracle(variant):
     """One C2 timestep at full size (8000 components, 8128 baselines, 768 channels) on the
     GPU; the oracle checks the first 8 channels, and linearity (model(A) + model(B) =
     model(A u B)) checks every channel."""
@@ -143,7 +153,7 @@ def test_c2_full_size_channel_slice_vs_oracle():
     v32, v64, _, m = gpu_model(obs, 3)
     o32, o64 = orc.model_timestep_slice(oracle_obs(obs, 3), obs.srclist, 0, 8)
     err = rel_err(v64[:8], o64)
-    print(f"C2 slice rel err {err:.3e}")
+    print(f"variant={variant} C2 slice rel err {err:.3e}")
     assert err <= TOL_F64
     items = list(obs.srclist.items())
     a = synthetic.shrink(obs)

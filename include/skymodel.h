@@ -129,7 +129,25 @@ typedef struct SmInfo {
     int32_t num_kernel_launches; /* direct modelling kernels timed since the last reset */
     double kernel_ms;            /* their summed duration */
     double kernel_ms_points, kernel_ms_gaussians, kernel_ms_shapelets;
+    double kernel_ms_factored;   /* split-precision MFMA point kernel */
+    int32_t num_factored_launches;
 } SmInfo;
+
+/* ------------------------------------------------------------------ split-precision points
+ * No counterpart in the reference. Models one timestep like sm_model_timestep_with_device,
+ * but point components go through the antenna-factorised MFMA path: per channel
+ * V^e = (K (.) C^e) K^H with K_pc = exp(i 2 pi (u_p l + v_p m + w_p (n - 1)) f / c), phases and
+ * their range reduction in FP64, operands split into hi + lo halves of `dtype`, products on
+ * MFMA with FP32 accumulation promoted to FP64 every 2048 components. Gaussians and
+ * shapelets take the FP64 direct kernels. Requires one beam shared by every tile (or no
+ * beam); with per-tile beams the points fall back to the FP64 direct kernel. */
+enum { SM_MFMA_FP16 = 0, SM_MFMA_BF16 = 1 };
+const char *sm_model_timestep_factored(SmModeller *m, double lst, double latitude, int32_t dtype,
+                                       JonesF32 *d_vis_fb, JonesF64 *d_vis_fb_f64);
+
+/* Self-test of the MFMA lane layout the factored kernel assumes: D (32 x 32 f32, row-major)
+ * = A (32 x 16 f16, row-major) . B (16 x 32 f16, row-major), one wave, device pointers. */
+const char *sm_selftest_mfma_layout(const void *d_a, const void *d_b, float *d_d);
 const char *sm_get_info(SmModeller *m, SmInfo *info);
 /* Enables (1) or disables (0) per-kernel HIP-event timing and resets the counters. */
 const char *sm_set_timing(SmModeller *m, int32_t enabled);

@@ -116,6 +116,8 @@ class SmInfo(C.Structure):
         ("kernel_ms_points", C.c_double),
         ("kernel_ms_gaussians", C.c_double),
         ("kernel_ms_shapelets", C.c_double),
+        ("kernel_ms_factored", C.c_double),
+        ("num_factored_launches", C.c_int32),
     ]
 
 
@@ -141,8 +143,8 @@ SIGNATURES = {
     "sm_synchronize": [],
     "sm_set_direct_variant": [_I],
     "sm_analytic_beam": [_I, _P, _P, _I, _P, _I, _P, _P, _P],
-    "sm_model_points_factored": [_P, _D, _D, _I, _P, _P],
-    "sm_factored_info": [_P, _P],
+    "sm_model_timestep_factored": [_P, _D, _D, _I, _P, _P],
+    "sm_selftest_mfma_layout": [_P, _P, _P],
     "sm_surrogate_forward": [_P, _I, _P, _P, _I],
     "sm_surrogate_train_step": [_P, _P, _P, _I, _P],
     "sm_surrogate_new": [_I, _I, _I, _I, C.POINTER(_P)],

@@ -7,6 +7,7 @@
 
 #include "sm_abi_common.h"
 #include "sm_tiled.h"
+#include "sm_factored.h"
 
 namespace sm {
 namespace {
@@ -207,6 +208,9 @@ struct SmModeller {
     // device
     DevBuf d_freqs, d_xyz, d_bl_tiles, d_bl_rows, d_tile_rows, d_freq_map, d_ufreqs, d_delays, d_gains, d_sbf;
     DevBuf d_uvw, d_vis32, d_vis64, d_accum, d_tile_uvw, d_pairs;
+    // split-precision point path
+    DevBuf d_fblocks, d_fblock_of, d_fout, d_fscratch;
+    int nfblocks = 0;
     int npairs = 0;
     std::vector<double> h_tile_uvw;
     DevType types[3];
@@ -219,7 +223,8 @@ struct SmModeller {
     std::vector<Timed> pending;
     std::vector<hipEvent_t> pool;
     int launches = 0;
-    double ms_total = 0, ms_type[3] = {0, 0, 0};
+    double ms_total = 0, ms_type[4] = {0, 0, 0, 0};
+    int launches_factored = 0;
     std::vector<UVW> h_uvw;
 };
 
@@ -512,6 +517,11 @@ extern "C" const char *sm_new(const SmObsParams *p, const SmComponent *comps, co
         const std::vector<int4> pairs = tile_pairs(m->nt);
         m->npairs = (int)pairs.size();
         SM_NEW_TRY(upload(m->d_pairs, pairs));
+        std::vector<int> block_of;
+        const std::vector<int2> fblocks = factored_blocks(m->nt, block_of);
+        m->nfblocks = (int)fblocks.size();
+        SM_NEW_TRY(upload(m->d_fblocks, fblocks));
+        SM_NEW_TRY(upload(m->d_fblock_of, block_of));
     }
 #undef SM_NEW_TRY
     // Reverse the source order (src/model/gpu.rs:230-241, src/model/cpu.rs:86-93).
@@ -544,8 +554,10 @@ extern "C" const char *sm_update_with_a_source(SmModeller *m, const SmComponent 
     return set_components(m, order.data(), (int)order.size());
 }
 
-extern "C" const char *sm_model_timestep_with_device(SmModeller *m, double lst, double latitude, const UVW *d_uvws,
-                                                     JonesF32 *d_vis_fb, JonesF64 *d_vis_fb_f64) {
+// One timestep on the device. factored_dtype < 0: FP64 direct kernels for every type;
+// otherwise points go through the split-precision MFMA path in that operand type.
+static const char *model_timestep_device(SmModeller *m, double lst, double latitude, const UVW *d_uvws,
+                                         JonesF32 *d_vis_fb, JonesF64 *d_vis_fb_f64, int factored_dtype) {
     if (m == nullptr || d_vis_fb == nullptr) return "null argument";
     if (m->nbl == 0) return nullptr;
     const UVW *uvw = d_uvws;
@@ -581,6 +593,8 @@ extern "C" const char *sm_model_timestep_with_device(SmModeller *m, double lst, 
         if (ty == SM_SHAPELET)
             if (const char *e = prepare_shapelet_uvs(m, d, lst)) return e;
         TypeInputs in = type_inputs(d);
+        const bool factored = ty == SM_POINT && factored_dtype >= 0 && ctx.beam.uniform_row >= 0 && m->nt >= 2 &&
+                              tile_uvw != nullptr;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (m->timing) {
             for (hipEvent_t *e : {&e0, &e1}) {
@@ -591,11 +605,51 @@ extern "C" const char *sm_model_timestep_with_device(SmModeller *m, double lst, 
                     SM_TRY(hipEventCreate(e));
                 }
             }
-            m->pending.push_back(SmModeller::Timed{e0, e1, ty});
+            m->pending.push_back(SmModeller::Timed{e0, e1, factored ? 3 : ty});
         }
-        SM_TRY(run_type(kinds[ty], in, ctx, d.coh, d_vis_fb, d_vis_fb_f64, 0, e0, e1));
+        if (!factored) {
+            SM_TRY(run_type(kinds[ty], in, ctx, d.coh, d_vis_fb, d_vis_fb_f64, 0, e0, e1));
+            continue;
+        }
+        // Coherency J S J^H for the shared beam row, then the MFMA path.
+        const int n = d.h.n();
+        SM_TRY(d.coh.reserve(sizeof(double) * 8 * (size_t)n * m->nchan_pad));
+        SM_TRY(launch_coherency(n, m->nchan, m->nchan_pad, ctx.d_freqs_pad, in.fb, ctx.beam, true, d.coh.as<double>(),
+                                0));
+        FactoredArgs fa;
+        fa.ntiles = m->nt;
+        fa.nchan = m->nchan;
+        fa.nchan_pad = m->nchan_pad;
+        fa.ncomp = n;
+        fa.nblocks = m->nfblocks;
+        fa.seg_steps = 128;
+        fa.blocks = m->d_fblocks.as<int2>();
+        fa.freqs = ctx.d_freqs_pad;
+        fa.tile_uvw = tile_uvw;
+        fa.lmn = in.lmn;
+        fa.coh = d.coh.as<double>();
+        fa.scale = nullptr;
+        SM_TRY(m->d_fout.reserve(sizeof(double) * factored_out_doubles(m->nchan, m->nfblocks)));
+        fa.out = m->d_fout.as<double>();
+        SM_TRY(m->d_fscratch.reserve(64));
+        unsigned long long *bits = m->d_fscratch.as<unsigned long long>();
+        double *scale = reinterpret_cast<double *>(bits + 1);
+        SM_TRY(launch_factored(factored_dtype, fa, d.coh.as<double>(), (size_t)n * m->nchan_pad * 8, bits, scale,
+                               m->d_bl_tiles.as<int2>(), m->nbl, m->d_fblock_of.as<int>(), d_vis_fb, d_vis_fb_f64, 0,
+                               e0, e1));
     }
     return nullptr;
+}
+
+extern "C" const char *sm_model_timestep_with_device(SmModeller *m, double lst, double latitude, const UVW *d_uvws,
+                                                     JonesF32 *d_vis_fb, JonesF64 *d_vis_fb_f64) {
+    return model_timestep_device(m, lst, latitude, d_uvws, d_vis_fb, d_vis_fb_f64, -1);
+}
+
+extern "C" const char *sm_model_timestep_factored(SmModeller *m, double lst, double latitude, int32_t dtype,
+                                                  JonesF32 *d_vis_fb, JonesF64 *d_vis_fb_f64) {
+    if (dtype != SM_MFMA_FP16 && dtype != SM_MFMA_BF16) return "dtype must be SM_MFMA_FP16 or SM_MFMA_BF16";
+    return model_timestep_device(m, lst, latitude, nullptr, d_vis_fb, d_vis_fb_f64, dtype);
 }
 
 extern "C" const char *sm_model_timestep_with(SmModeller *m, double lst, double latitude, UVW *uvws_out,
@@ -666,7 +720,10 @@ static const char *resolve_timing(SmModeller *m) {
         SM_TRY(hipEventElapsedTime(&ms, t.e0, t.e1));
         m->ms_type[t.type] += ms;
         m->ms_total += ms;
-        m->launches++;
+        if (t.type == 3)
+            m->launches_factored++;
+        else
+            m->launches++;
         m->pool.push_back(t.e0);
         m->pool.push_back(t.e1);
     }
@@ -689,6 +746,8 @@ extern "C" const char *sm_get_info(SmModeller *m, SmInfo *info) {
     info->kernel_ms_points = m->ms_type[0];
     info->kernel_ms_gaussians = m->ms_type[1];
     info->kernel_ms_shapelets = m->ms_type[2];
+    info->kernel_ms_factored = m->ms_type[3];
+    info->num_factored_launches = m->launches_factored;
     return nullptr;
 }
 
@@ -698,7 +757,8 @@ extern "C" const char *sm_set_timing(SmModeller *m, int32_t enabled) {
     m->timing = enabled != 0;
     m->launches = 0;
     m->ms_total = 0;
-    m->ms_type[0] = m->ms_type[1] = m->ms_type[2] = 0;
+    m->ms_type[0] = m->ms_type[1] = m->ms_type[2] = m->ms_type[3] = 0;
+    m->launches_factored = 0;
     return nullptr;
 }
 

@@ -1,0 +1,28 @@
+// Split-precision MFMA point path: arguments and launchers (factored.hip).
+#pragma once
+
+#include <vector>
+
+#include "sm_internal.h"
+
+namespace sm {
+
+struct FactoredArgs {
+    int ntiles, nchan, nchan_pad, ncomp, nblocks, seg_steps;
+    const int2 *blocks;       // [nblocks] (p-block, q-block), p-block <= q-block, 64 tiles each
+    const double *freqs;      // [nchan_pad]
+    const double *tile_uvw;   // [ntiles][3]
+    const double4 *lmn;       // [ncomp]
+    const double *coh;        // [ncomp][nchan_pad][8], J S J^H
+    const double *scale;      // device scalar, set by launch_factored
+    double *out;              // [nchan][nblocks][4][64][128] FP64 block results
+};
+
+std::vector<int2> factored_blocks(int ntiles, std::vector<int> &block_of);
+size_t factored_out_doubles(int nchan, int nblocks);
+hipError_t launch_factored(int dtype, FactoredArgs a, const double *coh_for_scale, size_t coh_n,
+                           unsigned long long *d_max_bits, double *d_scale, const int2 *bl_tiles, int nbl,
+                           const int *block_of, JonesF32 *vis32, JonesF64 *vis64, hipStream_t stream,
+                           hipEvent_t ev0, hipEvent_t ev1);
+
+}  // namespace sm

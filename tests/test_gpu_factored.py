@@ -1,0 +1,96 @@
+"""Split-precision MFMA point path (sm_model_timestep_factored).
+
+Bar: relative visibility error <= 1e-6 of max|V| against the FP64 oracle for fp16 hi+lo
+operands. bf16 hi+lo keeps 16 mantissa bits; its error is measured and reported, with a
+looser bound written here (5e-6). Per-tile beams fall back to the FP64 direct kernel and
+must then meet the FP64 bar."""
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from mwa_hyperdrive_amd import _lib, synthetic
+from mwa_hyperdrive_amd.modeller import AnalyticBeam, NoBeam, SkyModeller
+from tests.helpers import oracle_model, rel_err
+
+pytestmark = pytest.mark.gpu
+
+FP16_TOL = 1e-6
+BF16_TOL = 5e-6
+
+
+def test_mfma_layout_selftest():
+    rng = np.random.default_rng(0)
+    a = rng.integers(-8, 8, size=(32, 16)).astype(np.float16)
+    b = rng.integers(-8, 8, size=(16, 32)).astype(np.float16)
+    da = torch.tensor(a, device="cuda")
+    db = torch.tensor(b, device="cuda")
+    dd = torch.zeros((32, 32), dtype=torch.float32, device="cuda")
+    _lib.call("sm_selftest_mfma_layout", C.c_void_p(da.data_ptr()), C.c_void_p(db.data_ptr()),
+              C.c_void_p(dd.data_ptr()))
+    np.testing.assert_array_equal(dd.cpu().numpy(), a.astype(np.float32) @ b.astype(np.float32))
+
+
+def factored(obs, t, dtype):
+    beam = AnalyticBeam(obs.delays, obs.gains) if obs.beam == "analytic" else NoBeam()
+    m = SkyModeller(beam, obs.srclist, 0, obs.xyzs, obs.freqs, [], obs.phase_centre, 0.0, obs.latitude)
+    shape = (len(obs.freqs), obs.num_baselines, 8)
+    d32 = torch.zeros(shape, dtype=torch.float32, device="cuda")
+    d64 = torch.zeros(shape, dtype=torch.float64, device="cuda")
+    _lib.call("sm_model_timestep_factored", m._h, float(obs.lsts[t]), obs.latitude, dtype,
+              C.c_void_p(d32.data_ptr()), C.c_void_p(d64.data_ptr()))
+    torch.cuda.synchronize()
+    return d32.cpu().numpy(), d64.cpu().numpy(), m
+
+
+@pytest.mark.parametrize("beam", ["none", "analytic"])
+@pytest.mark.parametrize("n_tiles", [24, 70, 128])
+def test_factored_vs_oracle(beam, n_tiles):
+    """C3-like full-Stokes points; tile counts below, between and at 64-tile blocks."""
+    obs = synthetic.shrink(synthetic.config(3, scale=300 / 20000), tiles_n=n_tiles, chans=12, timesteps=2)
+    obs.beam = beam
+    if beam == "analytic":
+        obs.delays = np.zeros((n_tiles, 16), np.uint32)
+        obs.gains = np.ones((n_tiles, 32))
+    o32, o64 = oracle_model(obs, 1)
+    for dtype, tol, name in [(0, FP16_TOL, "fp16"), (1, BF16_TOL, "bf16")]:
+        v32, v64, m = factored(obs, 1, dtype)
+        err = rel_err(v64, o64)
+        print(f"{name} beam={beam} tiles={n_tiles}: rel err {err:.3e}")
+        assert err <= tol
+        assert m.info()["num_points"] == obs.num_components
+
+
+def test_factored_per_tile_beam_falls_back_to_fp64():
+    obs = synthetic.shrink(synthetic.config(3, scale=100 / 20000), tiles_n=20, chans=9, timesteps=1)
+    obs.beam = "analytic"
+    obs.delays = np.zeros((20, 16), np.uint32)
+    obs.gains = np.ones((20, 32))
+    obs.delays[4, 7] = 32
+    o32, o64 = oracle_model(obs, 0)
+    v32, v64, _ = factored(obs, 0, 0)
+    assert rel_err(v64, o64) <= 1e-12
+
+
+def test_factored_c3_full_timestep_vs_fp64_direct():
+    """One C3 timestep at full size (20 000 full-Stokes points, 8128 baselines, 768 channels):
+    split-precision MFMA against the FP64 direct kernel (itself within 1e-12 of the oracle)."""
+    obs = synthetic.config(3)
+    m = SkyModeller(NoBeam(), obs.srclist, 0, obs.xyzs, obs.freqs, [], obs.phase_centre, 0.0, obs.latitude)
+    shape = (len(obs.freqs), obs.num_baselines, 8)
+    ref = torch.zeros(shape, dtype=torch.float64, device="cuda")
+    d32 = torch.zeros(shape, dtype=torch.float32, device="cuda")
+    m.model_timestep_with_device(float(obs.lsts[5]), obs.latitude, d32, ref)
+    for dtype, tol, name in [(0, FP16_TOL, "fp16"), (1, BF16_TOL, "bf16")]:
+        d64 = torch.zeros(shape, dtype=torch.float64, device="cuda")
+        _lib.call("sm_model_timestep_factored", m._h, float(obs.lsts[5]), obs.latitude, dtype,
+                  C.c_void_p(d32.data_ptr()), C.c_void_p(d64.data_ptr()))
+        torch.cuda.synchronize()
+        diff = (d64 - ref).reshape(-1, 2)
+        scale = ref.reshape(-1, 2).norm(dim=1).max().item()
+        err = diff.norm(dim=1).max().item() / scale
+        print(f"C3 full timestep {name}: rel err vs FP64 direct {err:.3e}")
+        assert err <= tol

@@ -138,7 +138,7 @@ typedef struct SmInfo {
  * but point components go through the antenna-factorised MFMA path: per channel
  * V^e = (K (.) C^e) K^H with K_pc = exp(i 2 pi (u_p l + v_p m + w_p (n - 1)) f / c), phases and
  * their range reduction in FP64, operands split into hi + lo halves of `dtype`, products on
- * MFMA with FP32 accumulation promoted to FP64 every 2048 components. Gaussians and
+ * MFMA with FP32 accumulation promoted to FP64 every 512 components. Gaussians and
  * shapelets take the FP64 direct kernels. Requires one beam shared by every tile (or no
  * beam); with per-tile beams the points fall back to the FP64 direct kernel. */
 enum { SM_MFMA_FP16 = 0, SM_MFMA_BF16 = 1 };

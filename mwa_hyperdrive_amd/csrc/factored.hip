@@ -12,10 +12,11 @@
 //   * forms A = K C^e s (s a power-of-two scale keeping |A| inside the fp16 range) and
 //     the K operand, and splits each value x into hi = rn(x), lo = rn(x - hi) in the MFMA
 //     input type (fp16 or bf16), written to LDS,
-//   * runs hi*hi + hi*lo + lo*hi on v_mfma_f32_32x32x16_{f16,bf16} (2 x 2 output tiles of
-//     32 x 32 per wave, FP32 accumulation),
+//   * runs hi*hi, hi*lo, lo*hi and lo*lo on v_mfma_f32_32x32x16_{f16,bf16} (2 x 2 output
+//     tiles of 32 x 32 per wave), the hi*hi products into one FP32 accumulator and the three
+//     correction products into a second, so the corrections' roundings stay small,
 // and every `seg_steps` steps adds the FP32 partials into an FP64 buffer, so each FP32
-// accumulation spans at most 2048 components. A scatter kernel maps the FP64 block
+// accumulation spans at most 512 components. A scatter kernel maps the FP64 block
 // results to baselines and accumulates the visibilities.
 #include <cmath>
 #include <vector>
@@ -53,22 +54,24 @@ template <class E> __device__ __forceinline__ void split(float x, E &hi, E &lo) 
     lo = (E)(x - (float)hi);
 }
 
-template <class E> struct Vec4;
-template <> struct Vec4<_Float16> { typedef _Float16 type __attribute__((ext_vector_type(4))); };
-template <> struct Vec4<__bf16> { typedef __bf16 type __attribute__((ext_vector_type(4))); };
+template <class E> struct Vec2;
+template <> struct Vec2<_Float16> { typedef _Float16 type __attribute__((ext_vector_type(2))); };
+template <> struct Vec2<__bf16> { typedef __bf16 type __attribute__((ext_vector_type(2))); };
 
 // Eight waves: wave w computes Jones entry w % 4 for output columns (w / 4) * 64 .. + 63,
 // i.e. 2 x 2 tiles of 32 x 32. Generation: thread t handles tile t / 8 and components
-// 2 (t % 8), 2 (t % 8) + 1 of the step, writing 4 consecutive K elements per operand row.
+// 2 (t % 8), 2 (t % 8) + 1 of the step, writing the (re, im) pair of each per operand row.
 template <class E>
 __global__ __launch_bounds__(512) void k_factored(const FactoredArgs a) {
     typedef typename Mfma<E>::vec V;
-    typedef typename Vec4<E>::type V4;
+    typedef typename Vec2<E>::type V2;
     __shared__ __attribute__((aligned(16))) E s_ahi[4][kFB][kStride];
     __shared__ __attribute__((aligned(16))) E s_alo[4][kFB][kStride];
     __shared__ __attribute__((aligned(16))) E s_bhi[2 * kFB][kStride];
     __shared__ __attribute__((aligned(16))) E s_blo[2 * kFB][kStride];
     __shared__ double s_uvw[2 * kFB][3];
+    // The step's component data, staged once: 4 LMN doubles and 8 coherency doubles each.
+    __shared__ double s_cd[kFC][12];
 
     const int total = a.nchan * a.nblocks;
     int b = blockIdx.x;
@@ -96,83 +99,82 @@ __global__ __launch_bounds__(512) void k_factored(const FactoredArgs a) {
     const int gp = tid >> 3;          // tile handled in generation
     const int gc = (tid & 7) * 2;     // first of this thread's 2 components in the step
 
-    f16v acc[2][2];
+    // acc: hi*hi products; cor: the hi*lo + lo*hi corrections, about 2^-11 smaller, kept
+    // apart so their FP32 roundings do not land on the large partial sums.
+    f16v acc[2][2], cor[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
 #pragma unroll
-            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = cor[i][j][r] = 0.0f;
 
+    // Thread t < 192 fetches element t % 12 of component t / 12 of the next step while the
+    // current step's MFMAs run, and stores it to s_cd once they are issued.
+    auto fetch = [&](int step) -> double {
+        if (tid >= kFC * 12) return 0.0;
+        const int c = step * kFC + tid / 12;
+        const int e = tid % 12;
+        if (c >= a.ncomp) return 0.0;
+        if (e < 4) {
+            const double4 l = a.lmn[c];
+            return e == 0 ? l.x : (e == 1 ? l.y : (e == 2 ? l.z : 0.0));
+        }
+        return a.coh[((size_t)c * a.nchan_pad + f) * 8 + (e - 4)] * scale;
+    };
     const int nsteps = (a.ncomp + kFC - 1) / kFC;
+    if (tid < kFC * 12) s_cd[tid / 12][tid % 12] = fetch(0);
     for (int step = 0; step < nsteps; ++step) {
         const int c0 = step * kFC;
         __syncthreads();
-        {
-            V4 ah[4], al[4], bh0, bl0, bh1, bl1;
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                const int c = c0 + gc + j;
-                double kpr = 0.0, kpi = 0.0, kqr = 0.0, kqi = 0.0;
-                double ce[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-                if (c < a.ncomp) {
-                    const double4 l = a.lmn[c];
-                    const double thp = s_uvw[gp][0] * l.x + s_uvw[gp][1] * l.y + s_uvw[gp][2] * l.z;
-                    fast_sincos(thp * fc, kpi, kpr);
-                    if (diag) {
-                        kqr = kpr;
-                        kqi = kpi;
-                    } else {
-                        const double thq =
-                            s_uvw[kFB + gp][0] * l.x + s_uvw[kFB + gp][1] * l.y + s_uvw[kFB + gp][2] * l.z;
-                        fast_sincos(thq * fc, kqi, kqr);
-                    }
-                    const double *cc = a.coh + ((size_t)c * a.nchan_pad + f) * 8;
-#pragma unroll
-                    for (int e = 0; e < 8; ++e) ce[e] = cc[e] * scale;
+#pragma unroll 1
+        for (int j = 0; j < 2; ++j) {
+            const int cl = gc + j;
+            const int c = c0 + cl;
+            double kpr = 0.0, kpi = 0.0, kqr = 0.0, kqi = 0.0;
+            const double *sd = s_cd[cl];
+            if (c < a.ncomp) {
+                const double thp = s_uvw[gp][0] * sd[0] + s_uvw[gp][1] * sd[1] + s_uvw[gp][2] * sd[2];
+                fast_sincos(thp * fc, kpi, kpr);
+                if (diag) {
+                    kqr = kpr;
+                    kqi = kpi;
+                } else {
+                    const double thq = s_uvw[kFB + gp][0] * sd[0] + s_uvw[kFB + gp][1] * sd[1] +
+                                       s_uvw[kFB + gp][2] * sd[2];
+                    fast_sincos(thq * fc, kqi, kqr);
                 }
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const float are = (float)(kpr * ce[2 * e] - kpi * ce[2 * e + 1]);
-                    const float aim = (float)(kpr * ce[2 * e + 1] + kpi * ce[2 * e]);
-                    E h, l2;
-                    split<E>(are, h, l2);
-                    ah[e][2 * j] = h;
-                    al[e][2 * j] = l2;
-                    split<E>(aim, h, l2);
-                    ah[e][2 * j + 1] = h;
-                    al[e][2 * j + 1] = l2;
-                }
-                // Column gp carries K_q: rows (2c, 2c+1) hold (K_re, K_im) for V_re and
-                // (-K_im, K_re) for V_im.
-                const float qr = (float)kqr, qi = (float)kqi;
-                E h, l2;
-                split<E>(qr, h, l2);
-                bh0[2 * j] = h;
-                bl0[2 * j] = l2;
-                bh1[2 * j + 1] = h;
-                bl1[2 * j + 1] = l2;
-                split<E>(qi, h, l2);
-                bh0[2 * j + 1] = h;
-                bl0[2 * j + 1] = l2;
-                split<E>(-qi, h, l2);
-                bh1[2 * j] = h;
-                bl1[2 * j] = l2;
             }
+            const float pr = (float)kpr, pi = (float)kpi;
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                *reinterpret_cast<V4 *>(&s_ahi[e][gp][2 * gc]) = ah[e];
-                *reinterpret_cast<V4 *>(&s_alo[e][gp][2 * gc]) = al[e];
+                const float cr = c < a.ncomp ? (float)sd[4 + 2 * e] : 0.0f;
+                const float ci = c < a.ncomp ? (float)sd[5 + 2 * e] : 0.0f;
+                E h0, l0, h1, l1;
+                split<E>(pr * cr - pi * ci, h0, l0);
+                split<E>(pr * ci + pi * cr, h1, l1);
+                const V2 h = {h0, h1}, l2 = {l0, l1};
+                *reinterpret_cast<V2 *>(&s_ahi[e][gp][2 * cl]) = h;
+                *reinterpret_cast<V2 *>(&s_alo[e][gp][2 * cl]) = l2;
             }
-            *reinterpret_cast<V4 *>(&s_bhi[gp][2 * gc]) = bh0;
-            *reinterpret_cast<V4 *>(&s_blo[gp][2 * gc]) = bl0;
-            *reinterpret_cast<V4 *>(&s_bhi[kFB + gp][2 * gc]) = bh1;
-            *reinterpret_cast<V4 *>(&s_blo[kFB + gp][2 * gc]) = bl1;
+            // Column gp carries K_q: rows (2c, 2c+1) hold (K_re, K_im) for V_re and
+            // (-K_im, K_re) for V_im.
+            const float qr = (float)kqr, qi = (float)kqi;
+            E qrh, qrl, qih, qil, nqih, nqil;
+            split<E>(qr, qrh, qrl);
+            split<E>(qi, qih, qil);
+            split<E>(-qi, nqih, nqil);
+            const V2 h0 = {qrh, qih}, l0 = {qrl, qil}, h1 = {nqih, qrh}, l1 = {nqil, qrl};
+            *reinterpret_cast<V2 *>(&s_bhi[gp][2 * cl]) = h0;
+            *reinterpret_cast<V2 *>(&s_blo[gp][2 * cl]) = l0;
+            *reinterpret_cast<V2 *>(&s_bhi[kFB + gp][2 * cl]) = h1;
+            *reinterpret_cast<V2 *>(&s_blo[kFB + gp][2 * cl]) = l1;
         }
         __syncthreads();
+        const double next = step + 1 < nsteps ? fetch(step + 1) : 0.0;
         const int row = lane & 31;
         const int kofs = (lane >> 5) * 8;
-#pragma unroll
+#pragma unroll 1
         for (int kb = 0; kb < kFK / 16; ++kb) {
             V ahi[2], alo[2], bhi[2], blo[2];
 #pragma unroll
@@ -187,24 +189,28 @@ __global__ __launch_bounds__(512) void k_factored(const FactoredArgs a) {
 #pragma unroll
                 for (int ct = 0; ct < 2; ++ct) {
                     acc[rt][ct] = Mfma<E>::run(ahi[rt], bhi[ct], acc[rt][ct]);
-                    acc[rt][ct] = Mfma<E>::run(ahi[rt], blo[ct], acc[rt][ct]);
-                    acc[rt][ct] = Mfma<E>::run(alo[rt], bhi[ct], acc[rt][ct]);
+                    cor[rt][ct] = Mfma<E>::run(ahi[rt], blo[ct], cor[rt][ct]);
+                    cor[rt][ct] = Mfma<E>::run(alo[rt], bhi[ct], cor[rt][ct]);
+                    cor[rt][ct] = Mfma<E>::run(alo[rt], blo[ct], cor[rt][ct]);
                 }
         }
+        if (tid < kFC * 12) s_cd[tid / 12][tid % 12] = next;
         if ((step + 1) % a.seg_steps == 0 || step + 1 == nsteps) {
             // Output layout of v_mfma_f32_32x32x16: element r of lane l is row
             // 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32.
-            double *out = a.out + ((size_t)(f * a.nblocks + blk) * 4 + pol) * (kFB * 2 * kFB);
+            double *out = a.out + ((size_t)(f * a.nblocks + blk) * 4 + pol) * (kFB * 2 * kFB) +
+                          (4 * (lane >> 5)) * (2 * kFB) + colbase + (lane & 31);
 #pragma unroll
             for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
                 for (int ct = 0; ct < 2; ++ct)
 #pragma unroll
                     for (int r = 0; r < 16; ++r) {
-                        const int i = rt * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                        const int n = colbase + ct * 32 + (lane & 31);
-                        out[i * (2 * kFB) + n] += (double)acc[rt][ct][r];
+                        const int o = (rt * 32 + 8 * (r >> 2) + (r & 3)) * (2 * kFB) + ct * 32;
+                        out[o] += (double)acc[rt][ct][r] + (double)cor[rt][ct][r];
                         acc[rt][ct][r] = 0.0f;
+                        cor[rt][ct][r] = 0.0f;
+                        if ((r & 3) == 3) __builtin_amdgcn_sched_barrier(0);
                     }
         }
     }

@@ -622,7 +622,7 @@ static const char *model_timestep_device(SmModeller *m, double lst, double latit
         fa.nchan_pad = m->nchan_pad;
         fa.ncomp = n;
         fa.nblocks = m->nfblocks;
-        fa.seg_steps = 128;
+        fa.seg_steps = 32;
         fa.blocks = m->d_fblocks.as<int2>();
         fa.freqs = ctx.d_freqs_pad;
         fa.tile_uvw = tile_uvw;

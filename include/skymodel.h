@@ -145,6 +145,10 @@ enum { SM_MFMA_FP16 = 0, SM_MFMA_BF16 = 1 };
 const char *sm_model_timestep_factored(SmModeller *m, double lst, double latitude, int32_t dtype,
                                        JonesF32 *d_vis_fb, JonesF64 *d_vis_fb_f64);
 
+/* FP32 accumulation span of the split-precision kernel, in steps of 16 components (default
+ * 32 = 512 components): after each span the partial sums are added into FP64. Process-wide. */
+const char *sm_set_factored_segment(int32_t steps);
+
 /* Self-test of the MFMA lane layout the factored kernel assumes: D (32 x 32 f32, row-major)
  * = A (32 x 16 f16, row-major) . B (16 x 32 f16, row-major), one wave, device pointers. */
 const char *sm_selftest_mfma_layout(const void *d_a, const void *d_b, float *d_d);

@@ -145,6 +145,7 @@ SIGNATURES = {
     "sm_analytic_beam": [_I, _P, _P, _I, _P, _I, _P, _P, _P],
     "sm_model_timestep_factored": [_P, _D, _D, _I, _P, _P],
     "sm_selftest_mfma_layout": [_P, _P, _P],
+    "sm_set_factored_segment": [C.c_int32],
     "sm_surrogate_forward": [_P, _I, _P, _P, _I],
     "sm_surrogate_train_step": [_P, _P, _P, _I, _P],
     "sm_surrogate_new": [_I, _I, _I, _I, C.POINTER(_P)],

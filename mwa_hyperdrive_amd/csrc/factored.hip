@@ -6,18 +6,22 @@
 //   [V_re | V_im] = [A_re A_im] . [[K_re, -K_im], [K_im, K_re]],  A = K (.) C^e,
 // i.e. M = 4 Jones entries x T tiles, N = 2 x T, K = 2 Nc.
 //
-// A work-group (8 waves) owns one channel and one 64 x 64 block of tile pairs (p-block <=
-// q-block; blocks below the diagonal are never computed). Per step of 16 components it
-//   * evaluates the phases and their sincos in FP64 (theta f / c is reduced in FP64),
-//   * forms A = K C^e s (s a power-of-two scale keeping |A| inside the fp16 range) and
-//     the K operand, and splits each value x into hi = rn(x), lo = rn(x - hi) in the MFMA
-//     input type (fp16 or bf16), written to LDS,
-//   * runs hi*hi, hi*lo, lo*hi and lo*lo on v_mfma_f32_32x32x16_{f16,bf16} (2 x 2 output
-//     tiles of 32 x 32 per wave), the hi*hi products into one FP32 accumulator and the three
-//     correction products into a second, so the corrections' roundings stay small,
-// and every `seg_steps` steps adds the FP32 partials into an FP64 buffer, so each FP32
-// accumulation spans at most 512 components. A scatter kernel maps the FP64 block
-// results to baselines and accumulates the visibilities.
+// Two kernels per chunk of components:
+//   * k_factored_phasors evaluates every K_tc(f) once: the phase theta f / c and its range
+//     reduction in FP64 at the first channel of each 8-channel group, an FP64 rotation
+//     across the group's evenly spaced channels, stored as FP32 complex [f][tile][comp].
+//   * k_factored: a work-group (4 waves) owns one channel, one 64 x 64 block of tile pairs
+//     (p-block <= q-block; blocks below the diagonal are never computed) and two of the four
+//     Jones entries. Per step of 16 components it forms A = K C^e s (s a power-of-two scale
+//     keeping |A| inside the fp16 range) and the K operand, splits each value x into
+//     hi = rn(x), lo = rn(x - hi) in the MFMA input type (fp16 or bf16) in LDS, and runs
+//     hi*hi, hi*lo, lo*hi and lo*lo on v_mfma_f32_32x32x16_{f16,bf16} (2 x 2 output tiles of
+//     32 x 32 per wave): hi*hi into one FP32 accumulator, the three corrections into a
+//     second, so their roundings stay apart from the large partial sums. Every `seg_steps`
+//     steps the FP32 partials are added into an FP64 buffer. Two work-groups share a CU, so
+//     one group's operand generation overlaps the other's MFMAs.
+// A scatter kernel maps the FP64 block results to baselines and accumulates visibilities.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -34,6 +38,7 @@ constexpr int kFB = 64;             // tiles per block side
 constexpr int kFC = 16;             // components per step
 constexpr int kFK = 2 * kFC;        // real K per step
 constexpr int kStride = kFK + 8;    // LDS row stride in elements (16-byte aligned, staggered banks)
+constexpr size_t kKbufCap = size_t(4) << 30;  // phasor buffer bytes per chunk
 
 template <class E> struct Mfma;
 template <> struct Mfma<_Float16> {
@@ -54,53 +59,110 @@ template <class E> __device__ __forceinline__ void split(float x, E &hi, E &lo) 
     lo = (E)(x - (float)hi);
 }
 
-template <class E> struct Vec2;
-template <> struct Vec2<_Float16> { typedef _Float16 type __attribute__((ext_vector_type(2))); };
-template <> struct Vec2<__bf16> { typedef __bf16 type __attribute__((ext_vector_type(2))); };
+// K[f][t][c] for the chunk's components; tiles >= ntiles and components past the chunk
+// hold 0, so the MFMA kernel needs no bounds checks.
+__global__ __launch_bounds__(256) void k_factored_phasors(const FactoredArgs a) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (size_t)a.tpad * a.cpad) return;
+    const int cl = (int)(i % a.cpad);
+    const int t = (int)(i / a.cpad);
+    const int c = a.c_begin + cl;
+    const size_t fstride = (size_t)a.tpad * a.cpad;
+    float2 *dst = a.kbuf + i;
+    if (t >= a.ntiles || cl >= a.c_count || c >= a.ncomp) {
+        for (int f = 0; f < a.nchan; ++f) dst[f * fstride] = make_float2(0.0f, 0.0f);
+        return;
+    }
+    const double4 l = a.lmn[c];
+    const double th = a.tile_uvw[3 * t] * l.x + a.tile_uvw[3 * t + 1] * l.y + a.tile_uvw[3 * t + 2] * l.z;
+    for (int f0 = 0; f0 < a.nchan; f0 += SM_NCH) {
+        double s, co;
+        fast_sincos(th * (a.freqs[f0] / SM_VEL_C), s, co);
+        dst[f0 * fstride] = make_float2((float)co, (float)s);
+        if (a.even) {
+            double ds, dc;
+            fast_sincos(th * ((a.freqs[f0 + 1] - a.freqs[f0]) / SM_VEL_C), ds, dc);
+            for (int k = 1; k < SM_NCH && f0 + k < a.nchan; ++k) {
+                const double ns = s * dc + co * ds;
+                co = co * dc - s * ds;
+                s = ns;
+                dst[(f0 + k) * fstride] = make_float2((float)co, (float)s);
+            }
+        } else {
+            for (int k = 1; k < SM_NCH && f0 + k < a.nchan; ++k) {
+                fast_sincos(th * (a.freqs[f0 + k] / SM_VEL_C), s, co);
+                dst[(f0 + k) * fstride] = make_float2((float)co, (float)s);
+            }
+        }
+    }
+}
 
-// Eight waves: wave w computes Jones entry w % 4 for output columns (w / 4) * 64 .. + 63,
-// i.e. 2 x 2 tiles of 32 x 32. Generation: thread t handles tile t / 8 and components
-// 2 (t % 8), 2 (t % 8) + 1 of the step, writing the (re, im) pair of each per operand row.
+// cohf[f][c][8] = J S J^H * scale in FP32 for the chunk's components (0 past the chunk).
+__global__ __launch_bounds__(256) void k_factored_coh32(const FactoredArgs a) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (size_t)a.nchan * a.cpad * 8) return;
+    const int k = (int)(i & 7);
+    const size_t fc = i >> 3;
+    const int cl = (int)(fc % a.cpad);
+    const int f = (int)(fc / a.cpad);
+    float v = 0.0f;
+    if (cl < a.c_count) v = (float)(a.coh[((size_t)(a.c_begin + cl) * a.nchan_pad + f) * 8 + k] * *a.scale);
+    reinterpret_cast<float *>(a.cohf)[i] = v;
+}
+
+// A thread's operands for one step: K_p and K_q of its 4 components (FP32 complex pairs)
+// and their scaled coherencies for the work-group's two Jones entries.
+struct Ops {
+    float4 p[2], q[2], c[4];
+};
+
+// Four waves: wave w computes local Jones entry w % 2 (entry 2 * half + w % 2) for output
+// columns (w / 2) * 64 .. + 63 (V_re columns, then V_im columns), 2 x 2 tiles of 32 x 32.
+// Generation: thread t handles tile t / 4 of each block side and components 4 (t % 4) ..
+// 4 (t % 4) + 3 of the step, one 16-byte LDS row segment per operand.
 template <class E>
-__global__ __launch_bounds__(512) void k_factored(const FactoredArgs a) {
+__global__ __launch_bounds__(256, 2) void k_factored(const FactoredArgs a) {
     typedef typename Mfma<E>::vec V;
-    typedef typename Vec2<E>::type V2;
-    __shared__ __attribute__((aligned(16))) E s_ahi[4][kFB][kStride];
-    __shared__ __attribute__((aligned(16))) E s_alo[4][kFB][kStride];
+    __shared__ __attribute__((aligned(16))) E s_ahi[2][kFB][kStride];
+    __shared__ __attribute__((aligned(16))) E s_alo[2][kFB][kStride];
     __shared__ __attribute__((aligned(16))) E s_bhi[2 * kFB][kStride];
     __shared__ __attribute__((aligned(16))) E s_blo[2 * kFB][kStride];
-    __shared__ double s_uvw[2 * kFB][3];
-    // The step's component data, staged once: 4 LMN doubles and 8 coherency doubles each.
-    __shared__ double s_cd[kFC][12];
 
-    const int total = a.nchan * a.nblocks;
+    const int total = a.nchan * a.nblocks * 2;
     int b = blockIdx.x;
     if ((total & 7) == 0) b = (b & 7) * (total >> 3) + (b >> 3);
-    const int f = b / a.nblocks;
-    const int blk = b - f * a.nblocks;
+    const int f = b / (2 * a.nblocks);
+    const int rem = b - f * 2 * a.nblocks;
+    const int blk = rem >> 1;
+    const int half = rem & 1;
     const int2 pq = a.blocks[blk];
-    const bool diag = pq.x == pq.y;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    const int pol = wave & 3;
-    const int colbase = (wave >> 2) * kFB;
+    const int le = wave & 1;
+    const int colbase = (wave >> 1) * kFB;
 
-    if (tid < 2 * kFB) {
-        const int g = tid < kFB ? pq.x : pq.y;
-        int t = g * kFB + (tid & (kFB - 1));
-        if (t >= a.ntiles) t = 0;
-        s_uvw[tid][0] = a.tile_uvw[3 * t];
-        s_uvw[tid][1] = a.tile_uvw[3 * t + 1];
-        s_uvw[tid][2] = a.tile_uvw[3 * t + 2];
-    }
-    const double fc = a.freqs[f] / SM_VEL_C;
-    const double scale = *a.scale;
-    const int gp = tid >> 3;          // tile handled in generation
-    const int gc = (tid & 7) * 2;     // first of this thread's 2 components in the step
+    const int gr = tid >> 2;
+    const int gq = (tid & 3) * 4;
+    // Work-group-uniform bases with 32-bit per-thread offsets keep the addresses out of the
+    // vector registers. Diagonal blocks load K_q from the same rows as K_p.
+    const float2 *kf = a.kbuf + (size_t)f * a.tpad * a.cpad;
+    const unsigned offp = (unsigned)((pq.x * kFB + gr) * a.cpad + gq);
+    const unsigned offq = (unsigned)((pq.y * kFB + gr) * a.cpad + gq);
 
-    // acc: hi*hi products; cor: the hi*lo + lo*hi corrections, about 2^-11 smaller, kept
-    // apart so their FP32 roundings do not land on the large partial sums.
+    // Scaled FP32 coherencies of the thread's 4 components: (re, im) of the work-group's first
+    // Jones entry, then of its second, one float4 per component.
+    const float4 *cf = a.cohf + (size_t)f * a.cpad * 2 + gq * 2 + half;
+    auto load = [&](Ops &g, int step) {
+        const unsigned o = (unsigned)step * kFC;
+        g.p[0] = *reinterpret_cast<const float4 *>(kf + offp + o);
+        g.p[1] = *reinterpret_cast<const float4 *>(kf + offp + o + 2);
+        g.q[0] = *reinterpret_cast<const float4 *>(kf + offq + o);
+        g.q[1] = *reinterpret_cast<const float4 *>(kf + offq + o + 2);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) g.c[j] = cf[2 * (o + j)];
+    };
+
     f16v acc[2][2], cor[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -109,78 +171,69 @@ __global__ __launch_bounds__(512) void k_factored(const FactoredArgs a) {
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = cor[i][j][r] = 0.0f;
 
-    // Thread t < 192 fetches element t % 12 of component t / 12 of the next step while the
-    // current step's MFMAs run, and stores it to s_cd once they are issued.
-    auto fetch = [&](int step) -> double {
-        if (tid >= kFC * 12) return 0.0;
-        const int c = step * kFC + tid / 12;
-        const int e = tid % 12;
-        if (c >= a.ncomp) return 0.0;
-        if (e < 4) {
-            const double4 l = a.lmn[c];
-            return e == 0 ? l.x : (e == 1 ? l.y : (e == 2 ? l.z : 0.0));
-        }
-        return a.coh[((size_t)c * a.nchan_pad + f) * 8 + (e - 4)] * scale;
-    };
-    const int nsteps = (a.ncomp + kFC - 1) / kFC;
-    if (tid < kFC * 12) s_cd[tid / 12][tid % 12] = fetch(0);
-    for (int step = 0; step < nsteps; ++step) {
-        const int c0 = step * kFC;
+    const int nsteps = a.cpad / kFC;
+    const int row = lane & 31;
+    const int kofs = (lane >> 5) * 8;
+
+    // One step: operands from registers into LDS, then the MFMAs. The registers are
+    // reloaded with the step two ahead as soon as they are in LDS, so every global load has
+    // two steps of MFMAs to arrive; two register sets alternate.
+    auto run_step = [&](Ops &g, int step) {
         __syncthreads();
-#pragma unroll 1
-        for (int j = 0; j < 2; ++j) {
-            const int cl = gc + j;
-            const int c = c0 + cl;
-            double kpr = 0.0, kpi = 0.0, kqr = 0.0, kqi = 0.0;
-            const double *sd = s_cd[cl];
-            if (c < a.ncomp) {
-                const double thp = s_uvw[gp][0] * sd[0] + s_uvw[gp][1] * sd[1] + s_uvw[gp][2] * sd[2];
-                fast_sincos(thp * fc, kpi, kpr);
-                if (diag) {
-                    kqr = kpr;
-                    kqi = kpi;
-                } else {
-                    const double thq = s_uvw[kFB + gp][0] * sd[0] + s_uvw[kFB + gp][1] * sd[1] +
-                                       s_uvw[kFB + gp][2] * sd[2];
-                    fast_sincos(thq * fc, kqi, kqr);
-                }
-            }
-            const float pr = (float)kpr, pi = (float)kpi;
+        {
+            const float kr[4] = {g.p[0].x, g.p[0].z, g.p[1].x, g.p[1].z};
+            const float ki[4] = {g.p[0].y, g.p[0].w, g.p[1].y, g.p[1].w};
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const float cr = c < a.ncomp ? (float)sd[4 + 2 * e] : 0.0f;
-                const float ci = c < a.ncomp ? (float)sd[5 + 2 * e] : 0.0f;
-                E h0, l0, h1, l1;
-                split<E>(pr * cr - pi * ci, h0, l0);
-                split<E>(pr * ci + pi * cr, h1, l1);
-                const V2 h = {h0, h1}, l2 = {l0, l1};
-                *reinterpret_cast<V2 *>(&s_ahi[e][gp][2 * cl]) = h;
-                *reinterpret_cast<V2 *>(&s_alo[e][gp][2 * cl]) = l2;
+            for (int e = 0; e < 2; ++e) {
+                V h, l;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const float cr = e ? g.c[j].z : g.c[j].x;
+                    const float ci = e ? g.c[j].w : g.c[j].y;
+                    E h0, l0, h1, l1;
+                    split<E>(kr[j] * cr - ki[j] * ci, h0, l0);
+                    split<E>(kr[j] * ci + ki[j] * cr, h1, l1);
+                    h[2 * j] = h0;
+                    h[2 * j + 1] = h1;
+                    l[2 * j] = l0;
+                    l[2 * j + 1] = l1;
+                }
+                *reinterpret_cast<V *>(&s_ahi[e][gr][2 * gq]) = h;
+                *reinterpret_cast<V *>(&s_alo[e][gr][2 * gq]) = l;
             }
-            // Column gp carries K_q: rows (2c, 2c+1) hold (K_re, K_im) for V_re and
+            // Column gr carries K_q: rows (2c, 2c+1) hold (K_re, K_im) for V_re and
             // (-K_im, K_re) for V_im.
-            const float qr = (float)kqr, qi = (float)kqi;
-            E qrh, qrl, qih, qil, nqih, nqil;
-            split<E>(qr, qrh, qrl);
-            split<E>(qi, qih, qil);
-            split<E>(-qi, nqih, nqil);
-            const V2 h0 = {qrh, qih}, l0 = {qrl, qil}, h1 = {nqih, qrh}, l1 = {nqil, qrl};
-            *reinterpret_cast<V2 *>(&s_bhi[gp][2 * cl]) = h0;
-            *reinterpret_cast<V2 *>(&s_blo[gp][2 * cl]) = l0;
-            *reinterpret_cast<V2 *>(&s_bhi[kFB + gp][2 * cl]) = h1;
-            *reinterpret_cast<V2 *>(&s_blo[kFB + gp][2 * cl]) = l1;
+            const float qr[4] = {g.q[0].x, g.q[0].z, g.q[1].x, g.q[1].z};
+            const float qi[4] = {g.q[0].y, g.q[0].w, g.q[1].y, g.q[1].w};
+            V h0, l0, h1, l1;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                E rh, rl, ih, il;
+                split<E>(qr[j], rh, rl);
+                split<E>(qi[j], ih, il);
+                h0[2 * j] = rh;
+                h0[2 * j + 1] = ih;
+                l0[2 * j] = rl;
+                l0[2 * j + 1] = il;
+                h1[2 * j] = -ih;
+                h1[2 * j + 1] = rh;
+                l1[2 * j] = -il;
+                l1[2 * j + 1] = rl;
+            }
+            *reinterpret_cast<V *>(&s_bhi[gr][2 * gq]) = h0;
+            *reinterpret_cast<V *>(&s_blo[gr][2 * gq]) = l0;
+            *reinterpret_cast<V *>(&s_bhi[kFB + gr][2 * gq]) = h1;
+            *reinterpret_cast<V *>(&s_blo[kFB + gr][2 * gq]) = l1;
         }
         __syncthreads();
-        const double next = step + 1 < nsteps ? fetch(step + 1) : 0.0;
-        const int row = lane & 31;
-        const int kofs = (lane >> 5) * 8;
+        load(g, step + 2 < nsteps ? step + 2 : nsteps - 1);  // unconditional: no register moves
 #pragma unroll 1
         for (int kb = 0; kb < kFK / 16; ++kb) {
             V ahi[2], alo[2], bhi[2], blo[2];
 #pragma unroll
             for (int t2 = 0; t2 < 2; ++t2) {
-                ahi[t2] = *reinterpret_cast<const V *>(&s_ahi[pol][t2 * 32 + row][kb * 16 + kofs]);
-                alo[t2] = *reinterpret_cast<const V *>(&s_alo[pol][t2 * 32 + row][kb * 16 + kofs]);
+                ahi[t2] = *reinterpret_cast<const V *>(&s_ahi[le][t2 * 32 + row][kb * 16 + kofs]);
+                alo[t2] = *reinterpret_cast<const V *>(&s_alo[le][t2 * 32 + row][kb * 16 + kofs]);
                 bhi[t2] = *reinterpret_cast<const V *>(&s_bhi[colbase + t2 * 32 + row][kb * 16 + kofs]);
                 blo[t2] = *reinterpret_cast<const V *>(&s_blo[colbase + t2 * 32 + row][kb * 16 + kofs]);
             }
@@ -194,12 +247,15 @@ __global__ __launch_bounds__(512) void k_factored(const FactoredArgs a) {
                     cor[rt][ct] = Mfma<E>::run(alo[rt], blo[ct], cor[rt][ct]);
                 }
         }
-        if (tid < kFC * 12) s_cd[tid / 12][tid % 12] = next;
         if ((step + 1) % a.seg_steps == 0 || step + 1 == nsteps) {
             // Output layout of v_mfma_f32_32x32x16: element r of lane l is row
-            // 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32.
-            double *out = a.out + ((size_t)(f * a.nblocks + blk) * 4 + pol) * (kFB * 2 * kFB) +
-                          (4 * (lane >> 5)) * (2 * kFB) + colbase + (lane & 31);
+            // 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32. The work-group is the only
+            // writer of its region, so the FP64 adds go out as atomics that return nothing
+            // and need no wait. The empty asm keeps the compiler from hoisting 64 addresses
+            // out of the step loop.
+            double *out = a.out + ((size_t)(f * a.nblocks + blk) * 4 + 2 * half) * (kFB * 2 * kFB);
+            unsigned lo = (unsigned)(le * (kFB * 2 * kFB) + (4 * (lane >> 5)) * (2 * kFB) + colbase + (lane & 31));
+            asm volatile("" : "+v"(lo));
 #pragma unroll
             for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
@@ -207,12 +263,20 @@ __global__ __launch_bounds__(512) void k_factored(const FactoredArgs a) {
 #pragma unroll
                     for (int r = 0; r < 16; ++r) {
                         const int o = (rt * 32 + 8 * (r >> 2) + (r & 3)) * (2 * kFB) + ct * 32;
-                        out[o] += (double)acc[rt][ct][r] + (double)cor[rt][ct][r];
+                        unsafeAtomicAdd(&out[lo + o], (double)acc[rt][ct][r] + (double)cor[rt][ct][r]);
                         acc[rt][ct][r] = 0.0f;
                         cor[rt][ct][r] = 0.0f;
                         if ((r & 3) == 3) __builtin_amdgcn_sched_barrier(0);
                     }
         }
+    };
+
+    Ops g0, g1;
+    load(g0, 0);
+    load(g1, nsteps > 1 ? 1 : 0);
+    for (int step = 0; step < nsteps; step += 2) {
+        run_step(g0, step);
+        if (step + 1 < nsteps) run_step(g1, step + 1);
     }
 }
 
@@ -282,11 +346,30 @@ std::vector<int2> factored_blocks(int ntiles, std::vector<int> &block_of) {
 
 size_t factored_out_doubles(int nchan, int nblocks) { return (size_t)nchan * nblocks * 4 * kFB * 2 * kFB; }
 
-hipError_t launch_factored(int dtype, FactoredArgs a, const double *coh_for_scale, size_t coh_n,
+int factored_chunk(int nchan, int ntiles, int ncomp, int seg_steps) {
+    const size_t tpad = (size_t)((ntiles + kFB - 1) / kFB) * kFB;
+    const size_t per_comp = (size_t)nchan * tpad * sizeof(float2);
+    const int unit = kFC * std::max(seg_steps, 1);
+    size_t fit = kKbufCap / std::max<size_t>(per_comp, 1);
+    fit = std::max<size_t>(unit, fit / unit * unit);
+    const int all = (ncomp + kFC - 1) / kFC * kFC;
+    return (int)std::min<size_t>(fit, (size_t)std::max(all, kFC));
+}
+
+size_t factored_kbuf_bytes(int nchan, int ntiles, int chunk) {
+    const size_t tpad = (size_t)((ntiles + kFB - 1) / kFB) * kFB;
+    return (size_t)nchan * tpad * chunk * sizeof(float2);
+}
+
+size_t factored_cohf_bytes(int nchan, int chunk) { return (size_t)nchan * chunk * 8 * sizeof(float); }
+
+hipError_t launch_factored(int dtype, FactoredArgs a, int chunk, const double *coh_for_scale, size_t coh_n,
                            unsigned long long *d_max_bits, double *d_scale, const int2 *bl_tiles, int nbl,
                            const int *block_of, JonesF32 *vis32, JonesF64 *vis64, hipStream_t stream,
                            hipEvent_t ev0, hipEvent_t ev1) {
     if (a.ncomp <= 0 || a.nchan <= 0 || nbl <= 0) return hipSuccess;
+    if (chunk <= 0 || chunk % kFC != 0) return hipErrorInvalidValue;
+    a.tpad = (a.ntiles + kFB - 1) / kFB * kFB;
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, d_max_bits);
     hipLaunchKernelGGL(k_absmax, dim3(1024), dim3(256), 0, stream, coh_n, coh_for_scale, d_max_bits);
     hipLaunchKernelGGL(k_scale_from_max, dim3(1), dim3(1), 0, stream, d_max_bits, d_scale);
@@ -294,11 +377,20 @@ hipError_t launch_factored(int dtype, FactoredArgs a, const double *coh_for_scal
     const size_t nout = factored_out_doubles(a.nchan, a.nblocks);
     hipLaunchKernelGGL(k_fill, dim3(2048), dim3(256), 0, stream, nout, a.out);
     if (ev0 != nullptr) (void)hipEventRecord(ev0, stream);
-    const int total = a.nchan * a.nblocks;
-    if (dtype == SM_MFMA_BF16)
-        hipLaunchKernelGGL(k_factored<__bf16>, dim3(total), dim3(512), 0, stream, a);
-    else
-        hipLaunchKernelGGL(k_factored<_Float16>, dim3(total), dim3(512), 0, stream, a);
+    const int total = a.nchan * a.nblocks * 2;
+    for (int c0 = 0; c0 < a.ncomp; c0 += chunk) {
+        a.c_begin = c0;
+        a.c_count = std::min(chunk, a.ncomp - c0);
+        a.cpad = (a.c_count + kFC - 1) / kFC * kFC;
+        const size_t nph = (size_t)a.tpad * a.cpad;
+        hipLaunchKernelGGL(k_factored_phasors, dim3((unsigned)((nph + 255) / 256)), dim3(256), 0, stream, a);
+        const size_t ncf = (size_t)a.nchan * a.cpad * 8;
+        hipLaunchKernelGGL(k_factored_coh32, dim3((unsigned)((ncf + 255) / 256)), dim3(256), 0, stream, a);
+        if (dtype == SM_MFMA_BF16)
+            hipLaunchKernelGGL(k_factored<__bf16>, dim3(total), dim3(256), 0, stream, a);
+        else
+            hipLaunchKernelGGL(k_factored<_Float16>, dim3(total), dim3(256), 0, stream, a);
+    }
     if (ev1 != nullptr) (void)hipEventRecord(ev1, stream);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;

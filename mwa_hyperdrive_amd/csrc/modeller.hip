@@ -209,7 +209,7 @@ struct SmModeller {
     DevBuf d_freqs, d_xyz, d_bl_tiles, d_bl_rows, d_tile_rows, d_freq_map, d_ufreqs, d_delays, d_gains, d_sbf;
     DevBuf d_uvw, d_vis32, d_vis64, d_accum, d_tile_uvw, d_pairs;
     // split-precision point path
-    DevBuf d_fblocks, d_fblock_of, d_fout, d_fscratch;
+    DevBuf d_fblocks, d_fblock_of, d_fout, d_fscratch, d_fkbuf, d_fcohf;
     int nfblocks = 0;
     int npairs = 0;
     std::vector<double> h_tile_uvw;
@@ -554,6 +554,15 @@ extern "C" const char *sm_update_with_a_source(SmModeller *m, const SmComponent 
     return set_components(m, order.data(), (int)order.size());
 }
 
+// Steps of 16 components per FP32 accumulation span of the MFMA point kernel.
+static int g_factored_seg_steps = 32;
+
+extern "C" const char *sm_set_factored_segment(int32_t steps) {
+    if (steps < 1 || steps > 4096) return "segment must be 1..4096 steps";
+    g_factored_seg_steps = steps;
+    return nullptr;
+}
+
 // One timestep on the device. factored_dtype < 0: FP64 direct kernels for every type;
 // otherwise points go through the split-precision MFMA path in that operand type.
 static const char *model_timestep_device(SmModeller *m, double lst, double latitude, const UVW *d_uvws,
@@ -622,7 +631,8 @@ static const char *model_timestep_device(SmModeller *m, double lst, double latit
         fa.nchan_pad = m->nchan_pad;
         fa.ncomp = n;
         fa.nblocks = m->nfblocks;
-        fa.seg_steps = 32;
+        fa.seg_steps = g_factored_seg_steps;
+        fa.even = m->even;
         fa.blocks = m->d_fblocks.as<int2>();
         fa.freqs = ctx.d_freqs_pad;
         fa.tile_uvw = tile_uvw;
@@ -634,7 +644,12 @@ static const char *model_timestep_device(SmModeller *m, double lst, double latit
         SM_TRY(m->d_fscratch.reserve(64));
         unsigned long long *bits = m->d_fscratch.as<unsigned long long>();
         double *scale = reinterpret_cast<double *>(bits + 1);
-        SM_TRY(launch_factored(factored_dtype, fa, d.coh.as<double>(), (size_t)n * m->nchan_pad * 8, bits, scale,
+        const int chunk = factored_chunk(m->nchan, m->nt, n, fa.seg_steps);
+        SM_TRY(m->d_fkbuf.reserve(factored_kbuf_bytes(m->nchan, m->nt, chunk)));
+        fa.kbuf = m->d_fkbuf.as<float2>();
+        SM_TRY(m->d_fcohf.reserve(factored_cohf_bytes(m->nchan, chunk)));
+        fa.cohf = m->d_fcohf.as<float4>();
+        SM_TRY(launch_factored(factored_dtype, fa, chunk, d.coh.as<double>(), (size_t)n * m->nchan_pad * 8, bits, scale,
                                m->d_bl_tiles.as<int2>(), m->nbl, m->d_fblock_of.as<int>(), d_vis_fb, d_vis_fb_f64, 0,
                                e0, e1));
     }

@@ -9,18 +9,29 @@ namespace sm {
 
 struct FactoredArgs {
     int ntiles, nchan, nchan_pad, ncomp, nblocks, seg_steps;
+    int tpad;                 // ntiles rounded up to the 64-tile block size
+    int c_begin, c_count;     // the chunk of components in flight
+    int cpad;                 // c_count rounded up to the 16-component step
+    bool even;                // channels evenly spaced within each 8-channel group
     const int2 *blocks;       // [nblocks] (p-block, q-block), p-block <= q-block, 64 tiles each
     const double *freqs;      // [nchan_pad]
     const double *tile_uvw;   // [ntiles][3]
     const double4 *lmn;       // [ncomp]
     const double *coh;        // [ncomp][nchan_pad][8], J S J^H
     const double *scale;      // device scalar, set by launch_factored
+    float2 *kbuf;             // [nchan][tpad][cpad] phasors of the chunk
+    float4 *cohf;             // [nchan][cpad][2] scaled FP32 coherencies of the chunk
     double *out;              // [nchan][nblocks][4][64][128] FP64 block results
 };
 
 std::vector<int2> factored_blocks(int ntiles, std::vector<int> &block_of);
 size_t factored_out_doubles(int nchan, int nblocks);
-hipError_t launch_factored(int dtype, FactoredArgs a, const double *coh_for_scale, size_t coh_n,
+// Components per chunk (a multiple of 16 * seg_steps unless one chunk holds them all), so
+// that the phasor buffer stays within a fixed cap, and that buffer's size.
+int factored_chunk(int nchan, int ntiles, int ncomp, int seg_steps);
+size_t factored_kbuf_bytes(int nchan, int ntiles, int chunk);
+size_t factored_cohf_bytes(int nchan, int chunk);
+hipError_t launch_factored(int dtype, FactoredArgs a, int chunk, const double *coh_for_scale, size_t coh_n,
                            unsigned long long *d_max_bits, double *d_scale, const int2 *bl_tiles, int nbl,
                            const int *block_of, JonesF32 *vis32, JonesF64 *vis64, hipStream_t stream,
                            hipEvent_t ev0, hipEvent_t ev1);

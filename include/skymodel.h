@@ -171,6 +171,43 @@ const char *sm_analytic_beam(int32_t n_tiles, const uint32_t *delays, const doub
                              const double *freqs, int32_t n_dirs, const double *d_az, const double *d_za,
                              JonesF64 *d_out);
 
+/* ------------------------------------------------------------------ beam surrogate
+ * No counterpart in the reference: an MLP stand-in for the beam Jones (BeamGpu::calc_jones_pair,
+ * src/beam/mod.rs:143-177). 53 inputs (sin az, cos az, sin za, cos za, (f - 182 MHz) / 15 MHz,
+ * delays / 31 [16], gains [32]) -> 4 x 256 SiLU -> 8 outputs (J re/im, row-major). Parameters
+ * flat in the order W0 [256][53], b0 [256], W1 [256][256], b1, W2, b2, W3, b3, W4 [8][256], b4 [8]. */
+typedef struct SmSurrogate SmSurrogate;
+int32_t sm_surrogate_num_params(void);
+const char *sm_surrogate_new(uint64_t seed, SmSurrogate **out);
+void sm_surrogate_free(SmSurrogate *s);
+const char *sm_surrogate_get_params(SmSurrogate *s, float *host_params);
+const char *sm_surrogate_set_params(SmSurrogate *s, const float *host_params);
+/* Training samples from the exact (analytic) beam at seeded random inputs: az ~ U[0, 2 pi),
+ * cos za ~ U[0, 1], f ~ U[167, 197] MHz, delays ~ U{0..31}, gains 0 with probability 0.02.
+ * Sample i is a pure function of (seed, first + i). Device outputs (each nullable): features
+ * [n][53] f32, Jones [n][8] f32, raw inputs [n][51] f64 (az, za, f, delays[16], gains[32]). */
+const char *sm_surrogate_samples(uint64_t seed, int64_t first, int32_t n, float *d_x, float *d_y, double *d_raw);
+/* Mean-squared-error loss and its gradients (host, flat like the parameters) on a batch. */
+const char *sm_surrogate_gradients(SmSurrogate *s, int32_t n, const float *d_x, const float *d_y, double *loss,
+                                   float *host_grads);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on a batch; loss is the batch MSE before it. */
+const char *sm_surrogate_train_step(SmSurrogate *s, int32_t n, const float *d_x, const float *d_y, float lr,
+                                    double *loss);
+/* Forward pass: dtype 0 = FP32 (the training arithmetic), 1 = FP16 MFMA with FP32 accumulation. */
+const char *sm_surrogate_forward(SmSurrogate *s, int32_t n, const float *d_x, int32_t dtype, float *d_y);
+/* Beam Jones out[tile][freq][dir] like sm_analytic_beam, from the FP16 surrogate. Each tile is
+ * judged against the exact beam on up to 512 directions x 8 frequencies; a tile whose max
+ * |J_surrogate - J_exact| exceeds `threshold` takes the exact beam. n_fallback (nullable)
+ * receives the number of such tiles. */
+const char *sm_surrogate_beam(SmSurrogate *s, int32_t n_tiles, const uint32_t *delays, const double *gains,
+                              int32_t n_freqs, const double *freqs, int32_t n_dirs, const double *d_az,
+                              const double *d_za, double threshold, JonesF64 *d_out, int32_t *n_fallback);
+/* Makes the modeller take its beam Jones from the surrogate (with the exact beam as per-tile
+ * fallback above `threshold`), in place of the analytic beam; s = NULL restores the analytic
+ * beam. fallbacks_so_far (nullable) receives the number of tile fallbacks since the last call.
+ * The surrogate is not owned and must outlive its use. */
+const char *sm_set_beam_surrogate(SmModeller *m, SmSurrogate *s, double threshold, int32_t *fallbacks_so_far);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -146,13 +146,15 @@ SIGNATURES = {
     "sm_model_timestep_factored": [_P, _D, _D, _I, _P, _P],
     "sm_selftest_mfma_layout": [_P, _P, _P],
     "sm_set_factored_segment": [C.c_int32],
-    "sm_surrogate_forward": [_P, _I, _P, _P, _I],
-    "sm_surrogate_train_step": [_P, _P, _P, _I, _P],
-    "sm_surrogate_new": [_I, _I, _I, _I, C.POINTER(_P)],
-    "sm_surrogate_free": [_P],
-    "sm_surrogate_params": [_P, _P, _P, _P],
+    "sm_surrogate_new": [C.c_uint64, C.POINTER(_P)],
+    "sm_surrogate_get_params": [_P, _P],
     "sm_surrogate_set_params": [_P, _P],
-    "sm_beam_features": [_I, _P, _P, _D, _P, _P, _P, _I],
+    "sm_surrogate_samples": [C.c_uint64, C.c_int64, _I, _P, _P, _P],
+    "sm_surrogate_gradients": [_P, _I, _P, _P, C.POINTER(_D), _P],
+    "sm_surrogate_train_step": [_P, _I, _P, _P, C.c_float, C.POINTER(_D)],
+    "sm_surrogate_forward": [_P, _I, _P, _I, _P],
+    "sm_surrogate_beam": [_P, _I, _P, _P, _I, _P, _I, _P, _P, _D, _P, C.POINTER(_I)],
+    "sm_set_beam_surrogate": [_P, _P, _D, C.POINTER(_I)],
 }
 VOID_RETURN = {"sm_free", "sm_surrogate_free"}
 
@@ -170,8 +172,11 @@ def lib():
                 fn = getattr(L, name)
                 fn.argtypes = args
                 fn.restype = C.c_char_p
-        L.sm_free.argtypes = [_P]
-        L.sm_free.restype = None
+        for name in VOID_RETURN:
+            getattr(L, name).argtypes = [_P]
+            getattr(L, name).restype = None
+        L.sm_surrogate_num_params.argtypes = []
+        L.sm_surrogate_num_params.restype = C.c_int32
         _lib = L
     return _lib
 

@@ -196,6 +196,9 @@ struct SmModeller {
     std::vector<int> flagged;
     double pc_ra = 0, pc_dec = 0, lon = 0, lat = 0;
     int beam_type = SM_BEAM_NONE, pols = SM_POLS_XX_XY_YX_YY;
+    SmSurrogate *surrogate = nullptr;  // beam surrogate with exact fallback (not owned)
+    double surrogate_threshold = 0.0;
+    int surrogate_fallbacks = 0;       // tiles that took the exact beam, summed over calls
     std::vector<unsigned> delays;  // [n_total][16]
     std::vector<double> gains;     // [n_total][32]
     // unique tiles / freqs (src/model/cpu.rs:105-161)
@@ -386,8 +389,23 @@ const char *prepare_beam(SmModeller *m, DevType &d, double lst, double latitude,
     SM_TRY(upload(d.az, az));
     SM_TRY(upload(d.za, za));
     SM_TRY(d.beam.reserve(sizeof(JonesF64) * (size_t)m->n_utile * m->n_ufreq * n));
+    if (m->surrogate != nullptr) {
+        std::vector<uint32_t> dl;
+        std::vector<double> gn;
+        for (int u = 0; u < m->n_utile; ++u) {
+            const int t = m->unique_tile_total[u];
+            dl.insert(dl.end(), m->delays.begin() + 16 * t, m->delays.begin() + 16 * t + 16);
+            gn.insert(gn.end(), m->gains.begin() + 32 * t, m->gains.begin() + 32 * t + 32);
+        }
+        int32_t nfb = 0;
+        if (const char *e = sm_surrogate_beam(m->surrogate, m->n_utile, dl.data(), gn.data(), m->n_ufreq,
+                                              m->ufreqs.data(), n, d.az.as<double>(), d.za.as<double>(),
+                                              m->surrogate_threshold, d.beam.as<JonesF64>(), &nfb))
+            return e;
+        m->surrogate_fallbacks += nfb;
+    }
     // One launch per unique tile keeps the grid's y extent at n_ufreq.
-    for (int u = 0; u < m->n_utile; ++u) {
+    for (int u = 0; u < m->n_utile && m->surrogate == nullptr; ++u) {
         const int t = m->unique_tile_total[u];
         SM_TRY(launch_analytic_beam(1, m->d_delays.as<unsigned>() + 16 * t, m->d_gains.as<double>() + 32 * t,
                                     m->n_ufreq, m->d_ufreqs.as<double>(), n, d.az.as<double>(), d.za.as<double>(),
@@ -763,6 +781,17 @@ extern "C" const char *sm_get_info(SmModeller *m, SmInfo *info) {
     info->kernel_ms_shapelets = m->ms_type[2];
     info->kernel_ms_factored = m->ms_type[3];
     info->num_factored_launches = m->launches_factored;
+    return nullptr;
+}
+
+extern "C" const char *sm_set_beam_surrogate(SmModeller *m, SmSurrogate *s, double threshold,
+                                             int32_t *fallbacks_so_far) {
+    if (m == nullptr) return "null argument";
+    if (fallbacks_so_far != nullptr) *fallbacks_so_far = m->surrogate_fallbacks;
+    if (s != nullptr && m->beam_type != SM_BEAM_ANALYTIC) return "the surrogate replaces the analytic beam only";
+    m->surrogate = s;
+    m->surrogate_threshold = threshold;
+    m->surrogate_fallbacks = 0;
     return nullptr;
 }
 

@@ -44,6 +44,20 @@ def _dev_ptr(t):
     return None if t is None else C.c_void_p(t.data_ptr())
 
 
+def analytic_beam_device(delays, gains, freqs, d_az, d_za):
+    """The analytic beam on the device: torch float64 [tile][freq][dir][8] (sm_analytic_beam)."""
+    import torch
+
+    delays = np.ascontiguousarray(np.asarray(delays, np.uint32).reshape(-1, 16))
+    gains = np.ascontiguousarray(np.asarray(gains, np.float64).reshape(-1, 32))
+    freqs = np.ascontiguousarray(np.asarray(freqs, np.float64))
+    out = torch.empty((delays.shape[0], freqs.size, d_az.shape[0], 8), dtype=torch.float64, device=d_az.device)
+    _lib.call("sm_analytic_beam", delays.shape[0], C.c_void_p(delays.ctypes.data), C.c_void_p(gains.ctypes.data),
+              freqs.size, C.c_void_p(freqs.ctypes.data), d_az.shape[0], _dev_ptr(d_az), _dev_ptr(d_za),
+              _dev_ptr(out))
+    return out
+
+
 class SkyModeller:
     def __init__(self, beam, source_list, pols, unflagged_tile_xyzs, unflagged_fine_chan_freqs, flagged_tiles,
                  phase_centre, array_longitude_rad, array_latitude_rad):
@@ -132,6 +146,15 @@ class SkyModeller:
             vis_fb = np.zeros((self.num_freqs, self.num_tiles, 8), np.float32)
         _lib.call("sm_model_timestep_autos_with", self._h, lst, latitude, _ptr(vis_fb), _ptr(vis_fb_f64))
         return vis_fb
+
+    def set_beam_surrogate(self, surrogate, threshold: float) -> int:
+        """Beam Jones from the MLP surrogate with exact per-tile fallback above `threshold`
+        (None restores the analytic beam). Returns the fallbacks counted since the last call."""
+        n = C.c_int32()
+        _lib.call("sm_set_beam_surrogate", self._h, surrogate._h if surrogate is not None else None,
+                  float(threshold), C.byref(n))
+        self._surrogate = surrogate  # keep it alive while in use
+        return n.value
 
     # ---------------------------------------------------------------- measurement
     def set_timing(self, enabled: bool) -> None:

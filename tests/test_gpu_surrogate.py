@@ -1,0 +1,130 @@
+"""Beam surrogate on the GPU: samples, gradients, Adam, FP32 / FP16 forward passes, training
+and the exact-beam fallback, each checked against the CPU oracles (oracle/surrogate_oracle.py
+for the network, orc_analytic_beam_jones for the exact beam)."""
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from mwa_hyperdrive_amd import surrogate as sg
+from mwa_hyperdrive_amd.modeller import analytic_beam_device
+from oracle import oracle as orc
+from oracle import surrogate_oracle as so
+
+pytestmark = pytest.mark.gpu
+
+
+def test_samples_match_oracle_beam_and_features():
+    x, y, raw = sg.samples(11, 1000, 300, want_raw=True)
+    raw = raw.cpu().numpy()
+    np.testing.assert_allclose(x.cpu().numpy(), so.features(raw), rtol=1e-6, atol=1e-7)
+    exact = np.array([orc.analytic_beam(r[0], r[1], r[2], r[3:19].astype(np.uint32), r[19:51]) for r in raw])
+    np.testing.assert_allclose(y.cpu().numpy(), exact, rtol=1e-6, atol=1e-7)
+    # The distribution the generator promises.
+    assert (raw[:, 1] >= 0).all() and (raw[:, 1] <= np.pi / 2).all()
+    assert (raw[:, 2] >= 167e6).all() and (raw[:, 2] <= 197e6).all()
+    assert set(np.unique(raw[:, 3:19])) <= set(range(32))
+    # Same (seed, index) -> same sample, whatever the batch it is drawn in.
+    x2, _ = sg.samples(11, 1100, 5)
+    np.testing.assert_array_equal(x2.cpu().numpy(), x.cpu().numpy()[100:105])
+
+
+def test_gradients_match_oracle():
+    s = sg.BeamSurrogate(seed=5)
+    p = s.get_params().astype(np.float64)
+    x, y = sg.samples(3, 0, 512)
+    loss, g = s.gradients(x, y)
+    ref_loss, ref_g = so.loss_and_grads(p, x.cpu().numpy(), y.cpu().numpy())
+    assert abs(loss - ref_loss) <= 1e-5 * ref_loss
+    offs = np.cumsum([0] + [o * i + o for o, i in so.DIMS])
+    for li in range(len(so.DIMS)):
+        a, b = g[offs[li]:offs[li + 1]], ref_g[offs[li]:offs[li + 1]]
+        err = np.abs(a - b).max() / np.abs(b).max()
+        print(f"layer {li}: max grad err {err:.2e} of max |grad|")
+        assert err <= 1e-4
+
+
+def test_adam_step_matches_oracle():
+    s = sg.BeamSurrogate(seed=6)
+    p0 = s.get_params().astype(np.float64)
+    x, y = sg.samples(4, 0, 256)
+    _, g = s.gradients(x, y)
+    s.train_step(x, y, 1e-3)
+    p1 = s.get_params().astype(np.float64)
+    ref, _, _ = so.adam_step(p0, g.astype(np.float64), np.zeros_like(p0), np.zeros_like(p0), 1, 1e-3)
+    np.testing.assert_allclose(p1, ref, rtol=1e-6, atol=1e-7)
+
+
+def test_forward_fp32_and_fp16_vs_oracle():
+    s = sg.BeamSurrogate(seed=8)
+    p = s.get_params().astype(np.float64)
+    x, _ = sg.samples(9, 0, 1000)
+    ref, _ = so.forward(p, x.cpu().numpy())
+    scale = np.abs(ref).max()
+    y32 = s.forward(x, fp16=False).cpu().numpy()
+    y16 = s.forward(x, fp16=True).cpu().numpy()
+    e32 = np.abs(y32 - ref).max() / scale
+    e16 = np.abs(y16 - ref).max() / scale
+    print(f"forward vs oracle: fp32 {e32:.2e}, fp16 {e16:.2e} of max |y|")
+    assert e32 <= 1e-5
+    assert e16 <= 1e-2
+
+
+def test_training_reduces_error_and_fallback_is_exact():
+    s = sg.BeamSurrogate(seed=1)
+    xh, yh = sg.samples(99, 0, 20000)  # held out: another seed
+    def err():
+        d = s.forward(xh) - yh
+        return d.pow(2).mean().sqrt().item(), d.abs().max().item()
+    rms0, max0 = err()
+    info = s.train(n_samples=1 << 17, epochs=8, batch=4096, lr=2e-3)
+    rms1, after = err()
+    print(f"held-out |J err|: untrained rms {rms0:.3e} max {max0:.3e} -> trained rms {rms1:.3e} max {after:.3e} "
+          f"({info['steps']} steps, {info['seconds']:.1f} s)")
+    assert rms1 < rms0 / 5
+    # Fallback: threshold 0 sends every tile to the exact beam; a huge threshold keeps all.
+    rng = np.random.default_rng(2)
+    delays = np.zeros((3, 16), np.uint32)
+    delays[1] = rng.integers(0, 32, 16)
+    gains = np.ones((3, 32))
+    gains[2, 4] = 0.0
+    freqs = np.array([170e6, 180e6, 190e6])
+    az = torch.tensor(rng.uniform(0, 2 * np.pi, 700), device="cuda")
+    za = torch.tensor(np.arccos(rng.uniform(0, 1, 700)), device="cuda")
+    exact = analytic_beam_device(delays, gains, freqs, az, za)
+    j0, nfb0 = s.beam(delays, gains, freqs, az, za, threshold=0.0)
+    assert nfb0 == 3
+    assert torch.equal(j0, exact)
+    j1, nfb1 = s.beam(delays, gains, freqs, az, za, threshold=1e30)
+    assert nfb1 == 0
+    d = (j1 - exact).abs().max().item()
+    print(f"surrogate beam max |J err| on 3 tiles: {d:.3e}")
+    assert d < 10 * after + 1e-2
+
+
+def test_modeller_with_surrogate_beam():
+    """A C2-like mixed sky through the SkyModeller with the surrogate beam: threshold 0 gives
+    the analytic-beam visibilities bit for bit (every tile falls back); an infinite threshold
+    uses the network, whose visibility error is reported against the FP64 oracle."""
+    from mwa_hyperdrive_amd import synthetic
+    from mwa_hyperdrive_amd.modeller import AnalyticBeam, SkyModeller
+    from tests.helpers import oracle_model, rel_err
+
+    obs = synthetic.shrink(synthetic.config(2, scale=200 / 8000), tiles_n=24, chans=8, timesteps=1)
+    beam = AnalyticBeam(obs.delays, obs.gains)
+    m = SkyModeller(beam, obs.srclist, 0, obs.xyzs, obs.freqs, [], obs.phase_centre, 0.0, obs.latitude)
+    ref32, _, ref64 = m.model_timestep(float(obs.lsts[0]), obs.latitude, want_f64=True)
+    s = sg.BeamSurrogate(seed=1)
+    s.train(n_samples=1 << 16, epochs=4, batch=4096, lr=2e-3)
+    m.set_beam_surrogate(s, 0.0)
+    v32, _, v64 = m.model_timestep(float(obs.lsts[0]), obs.latitude, want_f64=True)
+    nfb = m.set_beam_surrogate(s, 1e30)
+    assert nfb >= 1
+    np.testing.assert_array_equal(v32, ref32)
+    w32, _, w64 = m.model_timestep(float(obs.lsts[0]), obs.latitude, want_f64=True)
+    assert m.set_beam_surrogate(None, 0.0) == 0
+    o32, o64 = oracle_model(obs, 0)
+    print(f"surrogate-beam sky: rel vis err vs oracle {rel_err(w64, o64):.3e}; exact beam {rel_err(ref64, o64):.3e}")
+    assert rel_err(ref64, o64) <= 1e-12
+    assert np.isfinite(w64).all()

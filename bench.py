@@ -1,13 +1,23 @@
 """Benchmark: sky-model visibility prediction on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+                    [--no-split-precision] [--no-surrogate]
 
-Workload (configuration C2): 128 tiles (8128 cross baselines), 768 channels x 40 kHz
-from 167 MHz, 14 timesteps of 8 s, 8000 components (5600 points, 2000 Gaussians,
+Headline workload (configuration C2): 128 tiles (8128 cross baselines), 768 channels x
+40 kHz from 167 MHz, 14 timesteps of 8 s, 8000 components (5600 points, 2000 Gaussians,
 400 shapelets) with power-law / curved / list fluxes, analytic beam with delays 0.
 One step = the whole job: every timestep modelled through the SkyModeller device entry
 (beam, coherency table, FP64 direct kernels), inputs resident in HBM, visibilities
-accumulated on the device. Prints one JSON line.
+accumulated on the device. `value` is that job's comp.vis/s.
+
+Also reported, outside the timed steps:
+  split_precision  C3 (20 000 full-Stokes points, 128 tiles, 768 ch, 14 timesteps): the FP64
+                   direct kernel against the fp16 / bf16 hi+lo MFMA path, all 14 timesteps
+                   each, with the error of one timestep against the FP64 path;
+  surrogate        the MLP beam: a short training run, held-out Jones error, and FP16
+                   inference against the exact beam on the C2 beam workload
+                   (1 tile x 768 ch x 8000 directions).
+Prints one JSON line.
 """
 
 from __future__ import annotations
@@ -62,6 +72,146 @@ def cpu_baseline(obs, chans: int, threads: int) -> dict:
     }
 
 
+FLOP_PER_COMPVIS_GAUSSIAN = FLOP_PER_COMPVIS_POINT + 21.0  # (kx, ky) 6, quadratic form 5, x K 1, exp 1, scale 8
+KERNEL_NAMES = {
+    "points": "k_direct_tiled<point> (FP64 tile-pair phasor kernel)",
+    "gaussians": "k_direct_tiled<gaussian> (FP64 tile-pair phasor kernel)",
+    "shapelets": "k_shapelet_chan (FP64 channel-per-lane shapelet kernel)",
+}
+
+
+def shapelet_flop(obs) -> float:
+    """Executed FP64 work per shapelet comp.vis: the point terms, 34 for the rotation,
+    positions and complex envelope, and 9 per coefficient (two table interpolations, the
+    product and the complex accumulation), at the sky's mean coefficient count."""
+    from mwa_hyperdrive_amd.srclist import Shapelet
+
+    counts = [len(c.comp_type.coeffs) for src in obs.srclist.values() for c in src.components
+              if isinstance(c.comp_type, Shapelet)]
+    mean = sum(counts) / len(counts) if counts else 0.0
+    return FLOP_PER_COMPVIS_POINT + 34.0 + 9.0 * mean
+
+
+def split_precision() -> dict:
+    """C3 points: FP64 direct vs the fp16 / bf16 hi+lo MFMA path over all 14 timesteps, each
+    timed with HIP events around its kernels; errors of timestep 5 against the FP64 path."""
+    import ctypes as C
+
+    import torch
+
+    from mwa_hyperdrive_amd import _lib, synthetic
+    from mwa_hyperdrive_amd.modeller import NoBeam, SkyModeller
+
+    obs = synthetic.config(3)
+    m = SkyModeller(NoBeam(), obs.srclist, 0, obs.xyzs, obs.freqs, [], obs.phase_centre, 0.0, obs.latitude)
+    shape = (len(obs.freqs), obs.num_baselines, 8)
+    v32 = torch.zeros(shape, dtype=torch.float32, device="cuda")
+    work = obs.num_components * obs.num_baselines * len(obs.freqs) * len(obs.lsts)
+    res = {"workload": f"C3: {obs.num_components} full-Stokes points, {obs.num_baselines} baselines, "
+                       f"{len(obs.freqs)} channels, {len(obs.lsts)} timesteps, no beam",
+           "comp_vis": work}
+
+    def run(kind, lst, d64=None):
+        if kind == "fp64":
+            m.model_timestep_with_device(lst, obs.latitude, v32, d64)
+        else:
+            _lib.call("sm_model_timestep_factored", m._h, lst, obs.latitude, kind, C.c_void_p(v32.data_ptr()),
+                      C.c_void_p(d64.data_ptr()) if d64 is not None else None)
+
+    ref = torch.zeros(shape, dtype=torch.float64, device="cuda")
+    run("fp64", float(obs.lsts[5]), ref)
+    torch.cuda.synchronize()
+    scale = ref.reshape(-1, 2).norm(dim=1).max().item()
+    for kind, name in (("fp64", "fp64_direct"), (0, "fp16_hi_lo"), (1, "bf16_hi_lo")):
+        run(kind, float(obs.lsts[0]))  # warm-up (allocations)
+        torch.cuda.synchronize()
+        m.set_timing(True)
+        t0 = time.perf_counter()
+        for lst in obs.lsts:
+            run(kind, float(lst))
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        info = m.info()
+        m.set_timing(False)
+        kern_ms = info["kernel_ms_points"] if kind == "fp64" else info["kernel_ms_factored"]
+        entry = {"kernel_ms": kern_ms, "wall_ms": wall * 1e3, "gcompvis_per_s_kernel": work / (kern_ms * 1e-3) / 1e9,
+                 "gcompvis_per_s_wall": work / wall / 1e9}
+        if kind != "fp64":
+            d64 = torch.zeros(shape, dtype=torch.float64, device="cuda")
+            run(kind, float(obs.lsts[5]), d64)
+            torch.cuda.synchronize()
+            entry["rel_err_vs_fp64"] = (d64 - ref).reshape(-1, 2).norm(dim=1).max().item() / scale
+            del d64
+        res[name] = entry
+        print(f"[bench] split precision {name}: {entry['gcompvis_per_s_kernel']:.1f} Gcomp.vis/s (kernels)",
+              file=sys.stderr, flush=True)
+    base = res["fp64_direct"]["gcompvis_per_s_kernel"]
+    for name in ("fp16_hi_lo", "bf16_hi_lo"):
+        res[name]["speedup_vs_fp64_direct"] = res[name]["gcompvis_per_s_kernel"] / base
+    # MFMA rate of the fp16 path: executed dense work (3 tile blocks x 4 Jones entries x
+    # 64 x 128 outputs x 2 K per component x 2 FLOP x 4 hi/lo products per channel).
+    blocks = 3
+    flop = blocks * 4 * 64 * 128 * 2 * obs.num_components * 2 * 4 * len(obs.freqs) * len(obs.lsts)
+    res["fp16_hi_lo"]["mfma_tflops_executed"] = flop / (res["fp16_hi_lo"]["kernel_ms"] * 1e-3) / 1e12
+    res["fp16_hi_lo"]["mfma_peak_tflops"] = 2500.0
+    del m, v32, ref
+    torch.cuda.empty_cache()
+    return res
+
+
+def surrogate_bench(obs) -> dict:
+    """Short training of the MLP beam, its held-out error, and FP16 inference against the exact
+    beam for the C2 beam workload (delays 0, 768 channels, 8000 directions)."""
+    import numpy as np
+    import torch
+
+    from mwa_hyperdrive_amd import surrogate as sg
+    from mwa_hyperdrive_amd.modeller import analytic_beam_device
+
+    s = sg.BeamSurrogate(seed=1)
+    tr = s.train(n_samples=1 << 20, epochs=4, batch=8192, lr=2e-3)
+    xh, yh = sg.samples(99, 0, 100000)
+    d = s.forward(xh) - yh
+    rms, mx = d.pow(2).mean().sqrt().item(), d.abs().max().item()
+    rng = np.random.default_rng(5)
+    n_dirs = 8000
+    az = torch.tensor(rng.uniform(0, 2 * np.pi, n_dirs), device="cuda")
+    za = torch.tensor(np.arccos(rng.uniform(0.3, 1.0, n_dirs)), device="cuda")
+    delays = np.zeros((1, 16), np.uint32)
+    gains = np.ones((1, 32))
+    freqs = np.asarray(obs.freqs, np.float64)
+    evals = n_dirs * freqs.size
+    exact = analytic_beam_device(delays, gains, freqs, az, za)
+    j, _ = s.beam(delays, gains, freqs, az, za, threshold=1e30)
+    torch.cuda.synchronize()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        analytic_beam_device(delays, gains, freqs, az, za)
+    torch.cuda.synchronize()
+    t_exact = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        s.beam(delays, gains, freqs, az, za, threshold=1e30)
+    torch.cuda.synchronize()
+    t_sur = (time.perf_counter() - t0) / reps
+    err_c2 = (j - exact).abs().max().item()
+    res = {
+        "network": "53 -> 4 x 256 SiLU -> 8, FP32 training / FP16 MFMA inference",
+        "train": {"samples": 1 << 20, "epochs": 4, "batch": 8192, "steps": tr["steps"], "seconds": tr["seconds"]},
+        "heldout_jones_err": {"rms": rms, "max": mx, "samples": 100000},
+        "c2_beam": {"evaluations": evals, "max_jones_err": err_c2,
+                    "exact_ms": t_exact * 1e3, "surrogate_ms": t_sur * 1e3,
+                    "exact_evals_per_s": evals / t_exact, "surrogate_evals_per_s": evals / t_sur,
+                    "surrogate_speedup": t_exact / t_sur},
+        "note": "exact = this build's analytic beam (about 6 FP64 sincos per evaluation); the FEE beam the "
+                "surrogate targets is unavailable offline",
+    }
+    print(f"[bench] surrogate: held-out rms {rms:.2e}, exact {t_exact * 1e3:.2f} ms vs surrogate "
+          f"{t_sur * 1e3:.2f} ms for {evals} evaluations", file=sys.stderr, flush=True)
+    return res
+
+
 def load_traffic(workload: str):
     p = ROOT / "profiles" / "traffic.json"
     if not p.exists():
@@ -80,6 +230,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-chans", type=int, default=16)
     ap.add_argument("--timesteps", type=int, default=0, help="limit timesteps per step (profiling runs only)")
+    ap.add_argument("--no-split-precision", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -128,10 +280,18 @@ def main() -> int:
     work = obs.num_components * obs.num_baselines * len(obs.freqs) * nts
     value = work * args.steps / elapsed / 1e9
     launches_per_type = args.steps * nts
-    ms_points = info["kernel_ms_points"] / launches_per_type
-    flop_launch = FLOP_PER_COMPVIS_POINT * info["num_points"] * obs.num_baselines * len(obs.freqs)
-    achieved = flop_launch / (ms_points * 1e-3) / 1e12
-    traffic = load_traffic("C2_points")
+    nbf = obs.num_baselines * len(obs.freqs)
+    per_kernel = {}
+    for ty, key, flop in (("points", "num_points", FLOP_PER_COMPVIS_POINT),
+                          ("gaussians", "num_gaussians", FLOP_PER_COMPVIS_GAUSSIAN),
+                          ("shapelets", "num_shapelets", shapelet_flop(obs))):
+        ms = info[f"kernel_ms_{ty}"] / launches_per_type
+        fl = flop * info[key] * nbf
+        per_kernel[ty] = {"avg_launch_ms": ms, "flop_per_comp_vis": flop,
+                          "achieved_tflops": fl / (ms * 1e-3) / 1e12 if ms > 0 else None}
+    dom = max(per_kernel, key=lambda k: per_kernel[k]["avg_launch_ms"])
+    achieved = per_kernel[dom]["achieved_tflops"]
+    traffic = load_traffic(f"C2_{dom}")
     out = {
         "metric": METRIC,
         "value": value,
@@ -154,21 +314,28 @@ def main() -> int:
         },
         "roofline": {
             "bound": "fp64",
-            "kernel": "k_direct<point> (FP64 direct point kernel)",
+            "kernel": KERNEL_NAMES[dom],
             "achieved": achieved,
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_TFLOPS,
             "traffic": traffic,
-            "avg_launch_ms": ms_points,
-            "flop_per_comp_vis": FLOP_PER_COMPVIS_POINT,
+            "avg_launch_ms": per_kernel[dom]["avg_launch_ms"],
+            "flop_per_comp_vis": per_kernel[dom]["flop_per_comp_vis"],
         },
+        "per_kernel": per_kernel,
         "kernels_ms_per_step": {
             "points": info["kernel_ms_points"] / args.steps,
             "gaussians": info["kernel_ms_gaussians"] / args.steps,
             "shapelets": info["kernel_ms_shapelets"] / args.steps,
         },
     }
+    del m, vis
+    torch.cuda.empty_cache()
+    if not args.no_split_precision:
+        out["split_precision"] = split_precision()
+    if not args.no_surrogate:
+        out["surrogate"] = surrogate_bench(obs)
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         print(f"[bench] CPU baseline: oracle on {threads} threads, {args.cpu_chans} channels", file=sys.stderr,

@@ -232,6 +232,9 @@ def main() -> int:
     ap.add_argument("--timesteps", type=int, default=0, help="limit timesteps per step (profiling runs only)")
     ap.add_argument("--no-split-precision", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--probe", action="store_true",
+                    help="counter runs only: one plain 1 GiB device copy (the calibration kernel) and one C2 "
+                         "timestep, no JSON")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -246,6 +249,20 @@ def main() -> int:
     from mwa_hyperdrive_amd.modeller import AnalyticBeam, SkyModeller
 
     torch.cuda.set_device(0)
+    if args.probe:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        obs = synthetic.config(2)
+        m = SkyModeller(AnalyticBeam(obs.delays, obs.gains), obs.srclist, 0, obs.xyzs, obs.freqs, [],
+                        obs.phase_centre, 0.0, obs.latitude)
+        vis = torch.zeros((len(obs.freqs), obs.num_baselines, 8), dtype=torch.float32, device="cuda")
+        m.model_timestep_with_device(float(obs.lsts[0]), obs.latitude, vis)
+        torch.cuda.synchronize()
+        print("[bench] probe done", file=sys.stderr, flush=True)
+        return 0
     obs = synthetic.config(2)
     nts = args.timesteps or len(obs.lsts)
     lsts = [float(x) for x in obs.lsts[:nts]]

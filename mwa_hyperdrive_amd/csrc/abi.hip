@@ -105,8 +105,12 @@ hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf
     a.beam = ctx.beam;
     a.vis32 = vis32;
     a.vis64 = vis64;
+    a.bl_subset = ctx.d_bl_subset;
+    a.nsub = ctx.nsub;
     if (ev0 != nullptr) (void)hipEventRecord(ev0, stream);
-    if (g_direct_variant == 0 && kind != KIND_SHAPELET && !per_tile && ctx.even && ctx.d_tile_uvw != nullptr &&
+    if (ctx.d_bl_subset != nullptr) {
+        e = launch_direct(kind, per_tile, ctx.even, a, stream);
+    } else if (g_direct_variant == 0 && kind != KIND_SHAPELET && !per_tile && ctx.even && ctx.d_tile_uvw != nullptr &&
         ctx.d_pairs != nullptr && ctx.ntiles >= 2) {
         TiledArgs t;
         t.ntiles = ctx.ntiles;

@@ -124,8 +124,11 @@ template <int KIND, bool PER_TILE, bool EVEN>
 __global__ __launch_bounds__(256) void k_direct(const DirectArgs a, int gx, int total) {
     int bx, chunk;
     logical_block(gx, total, bx, chunk);
+    // Lane -> baseline: all baselines, or the listed subset (outputs keep the full layout).
+    const int nlanes = a.bl_subset != nullptr ? a.nsub : a.nbl;
     const int bl_raw = bx * 256 + (int)threadIdx.x;
-    const int bl = bl_raw < a.nbl ? bl_raw : a.nbl - 1;
+    const int li = bl_raw < nlanes ? bl_raw : nlanes - 1;
+    const int bl = a.bl_subset != nullptr ? a.bl_subset[li] : li;
     const int f0 = chunk * SM_NCH;
 
     const double *__restrict__ coh = a.coh;
@@ -228,7 +231,7 @@ __global__ __launch_bounds__(256) void k_direct(const DirectArgs a, int gx, int 
             }
         }
     }
-    if (bl_raw >= a.nbl) return;
+    if (bl_raw >= nlanes) return;
 #pragma unroll
     for (int k = 0; k < SM_NCH; ++k) {
         const int f = f0 + k;
@@ -247,7 +250,7 @@ __global__ __launch_bounds__(256) void k_direct(const DirectArgs a, int gx, int 
 
 template <int KIND, bool PT>
 static hipError_t launch_direct_kind(const DirectArgs &a, bool even, hipStream_t stream) {
-    const int gx = (a.nbl + 255) / 256;
+    const int gx = ((a.bl_subset != nullptr ? a.nsub : a.nbl) + 255) / 256;
     const int gy = (a.nchan + SM_NCH - 1) / SM_NCH;
     const int total = gx * gy;
     if (even)
@@ -270,6 +273,7 @@ bool chunks_even(const double *freqs, int nchan) {
 
 hipError_t launch_direct(Kind kind, bool per_tile, bool even, const DirectArgs &a, hipStream_t stream) {
     if (a.ncomp <= 0 || a.nbl <= 0 || a.nchan <= 0) return hipSuccess;
+    if (a.bl_subset != nullptr && a.nsub <= 0) return hipSuccess;
     switch (kind) {
     case KIND_POINT:
         return per_tile ? launch_direct_kind<KIND_POINT, true>(a, even, stream)

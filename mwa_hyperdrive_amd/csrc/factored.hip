@@ -136,6 +136,9 @@ __global__ __launch_bounds__(256, 2) void k_factored(const FactoredArgs a) {
     const int blk = rem >> 1;
     const int half = rem & 1;
     const int2 pq = a.blocks[blk];
+    // In a diagonal block the output tile (rows 32..63, columns 0..31 of each half) holds only
+    // pairs p > q, which no baseline reads: its MFMAs and its flush are skipped.
+    const bool diag = pq.x == pq.y;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
@@ -241,6 +244,7 @@ __global__ __launch_bounds__(256, 2) void k_factored(const FactoredArgs a) {
             for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
                 for (int ct = 0; ct < 2; ++ct) {
+                    if (diag && rt == 1 && ct == 0) continue;
                     acc[rt][ct] = Mfma<E>::run(ahi[rt], bhi[ct], acc[rt][ct]);
                     cor[rt][ct] = Mfma<E>::run(ahi[rt], blo[ct], cor[rt][ct]);
                     cor[rt][ct] = Mfma<E>::run(alo[rt], bhi[ct], cor[rt][ct]);
@@ -259,7 +263,8 @@ __global__ __launch_bounds__(256, 2) void k_factored(const FactoredArgs a) {
 #pragma unroll
             for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
-                for (int ct = 0; ct < 2; ++ct)
+                for (int ct = 0; ct < 2; ++ct) {
+                    if (diag && rt == 1 && ct == 0) continue;
 #pragma unroll
                     for (int r = 0; r < 16; ++r) {
                         const int o = (rt * 32 + 8 * (r >> 2) + (r & 3)) * (2 * kFB) + ct * 32;
@@ -268,6 +273,7 @@ __global__ __launch_bounds__(256, 2) void k_factored(const FactoredArgs a) {
                         cor[rt][ct][r] = 0.0f;
                         if ((r & 3) == 3) __builtin_amdgcn_sched_barrier(0);
                     }
+                }
         }
     };
 
@@ -280,17 +286,17 @@ __global__ __launch_bounds__(256, 2) void k_factored(const FactoredArgs a) {
     }
 }
 
-// FP64 block results -> visibilities [freq][baseline] (+=), undoing the scale.
-__global__ void k_factored_scatter(const FactoredArgs a, const int2 *__restrict__ bl_tiles, int nbl,
+// FP64 block results -> visibilities [freq][baseline] (+=), undoing the scale. map[i] =
+// (baseline, p, q): the output baseline and its two tiles in the factored tile numbering.
+__global__ void k_factored_scatter(const FactoredArgs a, const int4 *__restrict__ map, int nmap, int nbl,
                                    const int *__restrict__ block_of, JonesF32 *vis32, JonesF64 *vis64) {
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (size_t)a.nchan * nbl) return;
-    const int f = (int)(idx / nbl);
-    const int bl = (int)(idx % nbl);
-    const int2 t = bl_tiles[bl];
+    if (idx >= (size_t)a.nchan * nmap) return;
+    const int f = (int)(idx / nmap);
+    const int4 e4 = map[idx % nmap];
     const int gb = (a.ntiles + kFB - 1) / kFB;
-    const int blk = block_of[(t.x / kFB) * gb + (t.y / kFB)];
-    const int pl = t.x % kFB, ql = t.y % kFB;
+    const int blk = block_of[(e4.y / kFB) * gb + (e4.z / kFB)];
+    const int pl = e4.y % kFB, ql = e4.z % kFB;
     const double inv = 1.0 / *a.scale;
     double v[8];
 #pragma unroll
@@ -299,14 +305,31 @@ __global__ void k_factored_scatter(const FactoredArgs a, const int2 *__restrict_
         v[2 * e] = o[ql] * inv;
         v[2 * e + 1] = o[kFB + ql] * inv;
     }
-    float *v32 = reinterpret_cast<float *>(&vis32[idx]);
+    const size_t vo = (size_t)f * nbl + e4.x;
+    float *v32 = reinterpret_cast<float *>(&vis32[vo]);
 #pragma unroll
     for (int e = 0; e < 8; ++e) v32[e] += (float)v[e];
     if (vis64 != nullptr) {
-        double *v64 = reinterpret_cast<double *>(&vis64[idx]);
+        double *v64 = reinterpret_cast<double *>(&vis64[vo]);
 #pragma unroll
         for (int e = 0; e < 8; ++e) v64[e] += v[e];
     }
+}
+
+// dst[i] = src[idx[i]] for rows of 3 doubles (tile UVWs of a tile subset).
+__global__ void k_gather_rows3(int n, const int *__restrict__ idx, const double *__restrict__ src, double *dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int t = idx[i];
+    dst[3 * i] = src[3 * t];
+    dst[3 * i + 1] = src[3 * t + 1];
+    dst[3 * i + 2] = src[3 * t + 2];
+}
+
+hipError_t launch_gather_rows3(int n, const int *d_idx, const double *d_src, double *d_dst, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_rows3, dim3((n + 255) / 256), dim3(256), 0, stream, n, d_idx, d_src, d_dst);
+    return hipGetLastError();
 }
 
 // max |coh| over the table (positive doubles order like their bit patterns).
@@ -364,10 +387,10 @@ size_t factored_kbuf_bytes(int nchan, int ntiles, int chunk) {
 size_t factored_cohf_bytes(int nchan, int chunk) { return (size_t)nchan * chunk * 8 * sizeof(float); }
 
 hipError_t launch_factored(int dtype, FactoredArgs a, int chunk, const double *coh_for_scale, size_t coh_n,
-                           unsigned long long *d_max_bits, double *d_scale, const int2 *bl_tiles, int nbl,
+                           unsigned long long *d_max_bits, double *d_scale, const int4 *map, int nmap, int nbl,
                            const int *block_of, JonesF32 *vis32, JonesF64 *vis64, hipStream_t stream,
                            hipEvent_t ev0, hipEvent_t ev1) {
-    if (a.ncomp <= 0 || a.nchan <= 0 || nbl <= 0) return hipSuccess;
+    if (a.ncomp <= 0 || a.nchan <= 0 || nmap <= 0) return hipSuccess;
     if (chunk <= 0 || chunk % kFC != 0) return hipErrorInvalidValue;
     a.tpad = (a.ntiles + kFB - 1) / kFB * kFB;
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, d_max_bits);
@@ -394,8 +417,8 @@ hipError_t launch_factored(int dtype, FactoredArgs a, int chunk, const double *c
     if (ev1 != nullptr) (void)hipEventRecord(ev1, stream);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const size_t n = (size_t)a.nchan * nbl;
-    hipLaunchKernelGGL(k_factored_scatter, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, a, bl_tiles, nbl,
+    const size_t n = (size_t)a.nchan * nmap;
+    hipLaunchKernelGGL(k_factored_scatter, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, a, map, nmap, nbl,
                        block_of, vis32, vis64);
     return hipGetLastError();
 }

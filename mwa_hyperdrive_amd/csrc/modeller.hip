@@ -212,8 +212,13 @@ struct SmModeller {
     DevBuf d_freqs, d_xyz, d_bl_tiles, d_bl_rows, d_tile_rows, d_freq_map, d_ufreqs, d_delays, d_gains, d_sbf;
     DevBuf d_uvw, d_vis32, d_vis64, d_accum, d_tile_uvw, d_pairs;
     // split-precision point path
-    DevBuf d_fblocks, d_fblock_of, d_fout, d_fscratch, d_fkbuf, d_fcohf;
+    DevBuf d_fblocks, d_fblock_of, d_fout, d_fscratch, d_fkbuf, d_fcohf, d_fmap;
     int nfblocks = 0;
+    // Per-tile beams: the tiles of the most common beam row (group 0) share one coherency
+    // J0 S J0^H, so their pairs take the MFMA path; every other baseline takes the FP64
+    // direct kernel through the baseline subset d_g0_rest.
+    int g0_row = -1, g0_n = 0, g0_nblocks = 0, g0_nmap = 0, g0_nrest = 0;
+    DevBuf d_g0_tiles, d_g0_blocks, d_g0_block_of, d_g0_map, d_g0_rest, d_g0_uvw, d_g0_coh;
     int npairs = 0;
     std::vector<double> h_tile_uvw;
     DevType types[3];
@@ -540,6 +545,46 @@ extern "C" const char *sm_new(const SmObsParams *p, const SmComponent *comps, co
         m->nfblocks = (int)fblocks.size();
         SM_NEW_TRY(upload(m->d_fblocks, fblocks));
         SM_NEW_TRY(upload(m->d_fblock_of, block_of));
+        std::vector<int4> fmap(m->nbl);
+        for (int b = 0; b < m->nbl; ++b) fmap[b] = make_int4(b, bl_tiles[b].x, bl_tiles[b].y, 0);
+        SM_NEW_TRY(upload(m->d_fmap, fmap));
+        if (m->n_utile > 1) {
+            // Group 0 = the beam row shared by the most tiles.
+            std::vector<int> count(m->n_utile, 0);
+            for (int t = 0; t < m->nt; ++t) count[m->tile_rows[t]]++;
+            int r0 = 0;
+            for (int r = 1; r < m->n_utile; ++r)
+                if (count[r] > count[r0]) r0 = r;
+            std::vector<int> g0_tiles, local(m->nt, -1);
+            for (int t = 0; t < m->nt; ++t)
+                if (m->tile_rows[t] == r0) {
+                    local[t] = (int)g0_tiles.size();
+                    g0_tiles.push_back(t);
+                }
+            if (g0_tiles.size() >= 2) {
+                std::vector<int4> g0_map;
+                std::vector<int> rest;
+                for (int b = 0; b < m->nbl; ++b) {
+                    const int i = bl_tiles[b].x, j = bl_tiles[b].y;
+                    if (local[i] >= 0 && local[j] >= 0)
+                        g0_map.push_back(make_int4(b, local[i], local[j], 0));
+                    else
+                        rest.push_back(b);
+                }
+                std::vector<int> g0_block_of;
+                const std::vector<int2> g0_blocks = factored_blocks((int)g0_tiles.size(), g0_block_of);
+                m->g0_row = r0;
+                m->g0_n = (int)g0_tiles.size();
+                m->g0_nblocks = (int)g0_blocks.size();
+                m->g0_nmap = (int)g0_map.size();
+                m->g0_nrest = (int)rest.size();
+                SM_NEW_TRY(upload(m->d_g0_tiles, g0_tiles));
+                SM_NEW_TRY(upload(m->d_g0_blocks, g0_blocks));
+                SM_NEW_TRY(upload(m->d_g0_block_of, g0_block_of));
+                SM_NEW_TRY(upload(m->d_g0_map, g0_map));
+                SM_NEW_TRY(upload(m->d_g0_rest, rest));
+            }
+        }
     }
 #undef SM_NEW_TRY
     // Reverse the source order (src/model/gpu.rs:230-241, src/model/cpu.rs:86-93).
@@ -620,10 +665,13 @@ static const char *model_timestep_device(SmModeller *m, double lst, double latit
         if (ty == SM_SHAPELET)
             if (const char *e = prepare_shapelet_uvs(m, d, lst)) return e;
         TypeInputs in = type_inputs(d);
-        const bool factored = ty == SM_POINT && factored_dtype >= 0 && ctx.beam.uniform_row >= 0 && m->nt >= 2 &&
-                              tile_uvw != nullptr;
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (m->timing) {
+        const bool have_uvw = tile_uvw != nullptr && m->nt >= 2;
+        const bool fact_uniform = ty == SM_POINT && factored_dtype >= 0 && ctx.beam.uniform_row >= 0 && have_uvw;
+        const bool fact_group = ty == SM_POINT && factored_dtype >= 0 && ctx.beam.uniform_row < 0 &&
+                                ctx.beam.jones != nullptr && m->g0_row >= 0 && have_uvw;
+        auto timed = [&](int type, hipEvent_t &e0, hipEvent_t &e1) -> const char * {
+            e0 = e1 = nullptr;
+            if (!m->timing) return nullptr;
             for (hipEvent_t *e : {&e0, &e1}) {
                 if (!m->pool.empty()) {
                     *e = m->pool.back();
@@ -632,44 +680,81 @@ static const char *model_timestep_device(SmModeller *m, double lst, double latit
                     SM_TRY(hipEventCreate(e));
                 }
             }
-            m->pending.push_back(SmModeller::Timed{e0, e1, factored ? 3 : ty});
-        }
-        if (!factored) {
+            m->pending.push_back(SmModeller::Timed{e0, e1, type});
+            return nullptr;
+        };
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (!fact_uniform && !fact_group) {
+            if (const char *e = timed(ty, e0, e1)) return e;
             SM_TRY(run_type(kinds[ty], in, ctx, d.coh, d_vis_fb, d_vis_fb_f64, 0, e0, e1));
             continue;
         }
-        // Coherency J S J^H for the shared beam row, then the MFMA path.
         const int n = d.h.n();
-        SM_TRY(d.coh.reserve(sizeof(double) * 8 * (size_t)n * m->nchan_pad));
-        SM_TRY(launch_coherency(n, m->nchan, m->nchan_pad, ctx.d_freqs_pad, in.fb, ctx.beam, true, d.coh.as<double>(),
-                                0));
+        const double *coh = nullptr;
+        const double *fuvw = tile_uvw;
+        int fnt = m->nt, fnblocks = m->nfblocks, nmap = m->nbl;
+        const int2 *fblocks = m->d_fblocks.as<int2>();
+        const int *fblock_of = m->d_fblock_of.as<int>();
+        const int4 *fmap = m->d_fmap.as<int4>();
+        if (fact_uniform) {
+            // Coherency J S J^H for the shared beam row.
+            SM_TRY(d.coh.reserve(sizeof(double) * 8 * (size_t)n * m->nchan_pad));
+            SM_TRY(launch_coherency(n, m->nchan, m->nchan_pad, ctx.d_freqs_pad, in.fb, ctx.beam, true,
+                                    d.coh.as<double>(), 0));
+            coh = d.coh.as<double>();
+        } else {
+            // Baselines outside group 0: FP64 direct with per-tile beams.
+            ModelCtx rest = ctx;
+            rest.d_bl_subset = m->d_g0_rest.as<int>();
+            rest.nsub = m->g0_nrest;
+            if (m->g0_nrest > 0) {
+                if (const char *e = timed(ty, e0, e1)) return e;
+                SM_TRY(run_type(kinds[ty], in, rest, d.coh, d_vis_fb, d_vis_fb_f64, 0, e0, e1));
+            }
+            // Group 0: coherency J0 S J0^H and its tiles' UVWs.
+            BeamDesc b0 = ctx.beam;
+            b0.uniform_row = m->g0_row;
+            SM_TRY(m->d_g0_coh.reserve(sizeof(double) * 8 * (size_t)n * m->nchan_pad));
+            SM_TRY(launch_coherency(n, m->nchan, m->nchan_pad, ctx.d_freqs_pad, in.fb, b0, true,
+                                    m->d_g0_coh.as<double>(), 0));
+            coh = m->d_g0_coh.as<double>();
+            SM_TRY(m->d_g0_uvw.reserve(sizeof(double) * 3 * m->g0_n));
+            SM_TRY(launch_gather_rows3(m->g0_n, m->d_g0_tiles.as<int>(), tile_uvw, m->d_g0_uvw.as<double>(), 0));
+            fuvw = m->d_g0_uvw.as<double>();
+            fnt = m->g0_n;
+            fnblocks = m->g0_nblocks;
+            nmap = m->g0_nmap;
+            fblocks = m->d_g0_blocks.as<int2>();
+            fblock_of = m->d_g0_block_of.as<int>();
+            fmap = m->d_g0_map.as<int4>();
+        }
+        if (const char *e = timed(3, e0, e1)) return e;
         FactoredArgs fa;
-        fa.ntiles = m->nt;
+        fa.ntiles = fnt;
         fa.nchan = m->nchan;
         fa.nchan_pad = m->nchan_pad;
         fa.ncomp = n;
-        fa.nblocks = m->nfblocks;
+        fa.nblocks = fnblocks;
         fa.seg_steps = g_factored_seg_steps;
         fa.even = m->even;
-        fa.blocks = m->d_fblocks.as<int2>();
+        fa.blocks = fblocks;
         fa.freqs = ctx.d_freqs_pad;
-        fa.tile_uvw = tile_uvw;
+        fa.tile_uvw = fuvw;
         fa.lmn = in.lmn;
-        fa.coh = d.coh.as<double>();
+        fa.coh = coh;
         fa.scale = nullptr;
-        SM_TRY(m->d_fout.reserve(sizeof(double) * factored_out_doubles(m->nchan, m->nfblocks)));
+        SM_TRY(m->d_fout.reserve(sizeof(double) * factored_out_doubles(m->nchan, fnblocks)));
         fa.out = m->d_fout.as<double>();
         SM_TRY(m->d_fscratch.reserve(64));
         unsigned long long *bits = m->d_fscratch.as<unsigned long long>();
         double *scale = reinterpret_cast<double *>(bits + 1);
-        const int chunk = factored_chunk(m->nchan, m->nt, n, fa.seg_steps);
-        SM_TRY(m->d_fkbuf.reserve(factored_kbuf_bytes(m->nchan, m->nt, chunk)));
+        const int chunk = factored_chunk(m->nchan, fnt, n, fa.seg_steps);
+        SM_TRY(m->d_fkbuf.reserve(factored_kbuf_bytes(m->nchan, fnt, chunk)));
         fa.kbuf = m->d_fkbuf.as<float2>();
         SM_TRY(m->d_fcohf.reserve(factored_cohf_bytes(m->nchan, chunk)));
         fa.cohf = m->d_fcohf.as<float4>();
-        SM_TRY(launch_factored(factored_dtype, fa, chunk, d.coh.as<double>(), (size_t)n * m->nchan_pad * 8, bits, scale,
-                               m->d_bl_tiles.as<int2>(), m->nbl, m->d_fblock_of.as<int>(), d_vis_fb, d_vis_fb_f64, 0,
-                               e0, e1));
+        SM_TRY(launch_factored(factored_dtype, fa, chunk, coh, (size_t)n * m->nchan_pad * 8, bits, scale, fmap, nmap,
+                               m->nbl, fblock_of, d_vis_fb, d_vis_fb_f64, 0, e0, e1));
     }
     return nullptr;
 }

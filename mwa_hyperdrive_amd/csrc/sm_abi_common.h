@@ -27,6 +27,8 @@ struct ModelCtx {
     const double *d_tile_uvw = nullptr;  // [ntiles][3], enables the tile-pair kernel
     const int4 *d_pairs = nullptr;
     int npairs = 0;
+    const int *d_bl_subset = nullptr;  // nullable: the direct kernels model only these baselines
+    int nsub = 0;
 };
 
 // 0: pick the fastest kernel for the case; 1: always the baseline-per-lane kernels.

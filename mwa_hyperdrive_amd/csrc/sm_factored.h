@@ -31,8 +31,11 @@ size_t factored_out_doubles(int nchan, int nblocks);
 int factored_chunk(int nchan, int ntiles, int ncomp, int seg_steps);
 size_t factored_kbuf_bytes(int nchan, int ntiles, int chunk);
 size_t factored_cohf_bytes(int nchan, int chunk);
+hipError_t launch_gather_rows3(int n, const int *d_idx, const double *d_src, double *d_dst, hipStream_t stream);
+// map[nmap] = (output baseline, p, q) with p < q in the factored tile numbering (a.ntiles
+// tiles); nbl = the visibility buffer's baseline count.
 hipError_t launch_factored(int dtype, FactoredArgs a, int chunk, const double *coh_for_scale, size_t coh_n,
-                           unsigned long long *d_max_bits, double *d_scale, const int2 *bl_tiles, int nbl,
+                           unsigned long long *d_max_bits, double *d_scale, const int4 *map, int nmap, int nbl,
                            const int *block_of, JonesF32 *vis32, JonesF64 *vis64, hipStream_t stream,
                            hipEvent_t ev0, hipEvent_t ev1);
 

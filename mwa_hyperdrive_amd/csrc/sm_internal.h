@@ -62,6 +62,8 @@ struct DirectArgs {
     BeamDesc beam;            // per-tile mode only
     JonesF32 *vis32;          // [nchan][nbl]
     JonesF64 *vis64;          // nullable
+    const int *bl_subset = nullptr;  // nullable: model only these baselines (nsub of them)
+    int nsub = 0;
 };
 
 // Launchers (direct.hip). All enqueue on `stream` and return hipSuccess or an error.

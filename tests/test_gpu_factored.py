@@ -64,15 +64,32 @@ def test_factored_vs_oracle(beam, n_tiles):
         assert m.info()["num_points"] == obs.num_components
 
 
-def test_factored_per_tile_beam_falls_back_to_fp64():
-    obs = synthetic.shrink(synthetic.config(3, scale=100 / 20000), tiles_n=20, chans=9, timesteps=1)
+@pytest.mark.parametrize("n_tiles,dead", [(20, [4]), (40, [3, 11, 25, 38]), (70, [0, 69])])
+def test_factored_per_tile_beams(n_tiles, dead):
+    """Per-tile beams (tiles with one dead dipole each, as in the combined 256-antenna case):
+    pairs inside the most common beam row take the MFMA path, every baseline that touches
+    another row takes the FP64 direct kernel and must meet the FP64 bar on its own."""
+    obs = synthetic.shrink(synthetic.config(3, scale=100 / 20000), tiles_n=n_tiles, chans=9, timesteps=1)
     obs.beam = "analytic"
-    obs.delays = np.zeros((20, 16), np.uint32)
-    obs.gains = np.ones((20, 32))
-    obs.delays[4, 7] = 32
+    obs.delays = np.zeros((n_tiles, 16), np.uint32)
+    obs.gains = np.ones((n_tiles, 32))
+    for k, t in enumerate(dead):
+        obs.delays[t, (3 * k + 7) % 16] = 32
     o32, o64 = oracle_model(obs, 0)
     v32, v64, _ = factored(obs, 0, 0)
-    assert rel_err(v64, o64) <= 1e-12
+    err = rel_err(v64, o64)
+    odd = np.zeros(obs.num_baselines, bool)
+    b = 0
+    for i in range(n_tiles):
+        for j in range(i + 1, n_tiles):
+            odd[b] = i in dead or j in dead
+            b += 1
+    scale = np.sqrt((o64.reshape(-1, 2) ** 2).sum(1)).max()
+    d = v64[:, odd] - o64[:, odd]
+    err_rest = np.sqrt((d.reshape(-1, 2) ** 2).sum(1)).max() / scale
+    print(f"per-tile beams, {n_tiles} tiles, dead {dead}: rel err {err:.3e} (FP64 baselines {err_rest:.3e})")
+    assert err <= FP16_TOL
+    assert err_rest <= 1e-12
 
 
 def test_factored_c3_full_timestep_vs_fp64_direct():

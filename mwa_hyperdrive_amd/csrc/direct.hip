@@ -88,26 +88,7 @@ __device__ __forceinline__ void shapelet_envelope(double x_pos, double y_pos, in
     const int ix = (int)fmin(fmax(fx, 0.0), (double)(SM_SBF_L - 2));
     const int iy = (int)fmin(fmax(fy, 0.0), (double)(SM_SBF_L - 2));
     const double wx = x_pos - fx, wy = y_pos - fy;
-    er = 0.0;
-    ei = 0.0;
-    for (int j = 0; j < range.y; ++j) {
-        const ShapeletCoeff cf = coeffs[range.x + j];
-        const int n1 = cf.n1 < SM_SBF_N ? cf.n1 : SM_SBF_N - 1;
-        const int n2 = cf.n2 < SM_SBF_N ? cf.n2 : SM_SBF_N - 1;
-        const double *r1 = sbf + (size_t)n1 * SM_SBF_L + ix;
-        const double *r2 = sbf + (size_t)n2 * SM_SBF_L + iy;
-        const double x_low = r1[0], x_high = r1[1];
-        const double y_low = r2[0], y_high = r2[1];
-        const double u_value = x_low + (x_high - x_low) * wx;
-        const double v_value = y_low + (y_high - y_low) * wy;
-        const double t = (cf.value * u_value) * v_value;
-        switch ((cf.n1 + cf.n2) & 3) {
-        case 0: er += t; break;
-        case 1: ei += t; break;
-        case 2: er -= t; break;
-        default: ei -= t; break;
-        }
-    }
+    shapelet_sum(coeffs + range.x, range.y, sbf, ix, iy, wx, wy, er, ei);
 }
 
 // ------------------------------------------------------------------ direct kernel

@@ -183,10 +183,13 @@ void sm_surrogate_free(SmSurrogate *s);
 const char *sm_surrogate_get_params(SmSurrogate *s, float *host_params);
 const char *sm_surrogate_set_params(SmSurrogate *s, const float *host_params);
 /* Training samples from the exact (analytic) beam at seeded random inputs: az ~ U[0, 2 pi),
- * cos za ~ U[0, 1], f ~ U[167, 197] MHz, delays ~ U{0..31}, gains 0 with probability 0.02.
- * Sample i is a pure function of (seed, first + i). Device outputs (each nullable): features
- * [n][53] f32, Jones [n][8] f32, raw inputs [n][51] f64 (az, za, f, delays[16], gains[32]). */
-const char *sm_surrogate_samples(uint64_t seed, int64_t first, int32_t n, float *d_x, float *d_y, double *d_raw);
+ * cos za ~ U[0, 1], f ~ U[167, 197] MHz, gains 0 with probability 0.02, and delays either
+ * independent U{0..31} (mode 0) or those pointing the tile within 60 deg of zenith (mode 1).
+ * Sample i is a pure function of (seed, mode, first + i). Device outputs (each nullable):
+ * features [n][53] f32, Jones [n][8] f32, raw inputs [n][51] f64 (az, za, f, delays[16],
+ * gains[32]). */
+const char *sm_surrogate_samples(uint64_t seed, int64_t first, int32_t n, int32_t mode, float *d_x, float *d_y,
+                                 double *d_raw);
 /* Mean-squared-error loss and its gradients (host, flat like the parameters) on a batch. */
 const char *sm_surrogate_gradients(SmSurrogate *s, int32_t n, const float *d_x, const float *d_y, double *loss,
                                    float *host_grads);

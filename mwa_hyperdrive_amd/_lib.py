@@ -149,7 +149,7 @@ SIGNATURES = {
     "sm_surrogate_new": [C.c_uint64, C.POINTER(_P)],
     "sm_surrogate_get_params": [_P, _P],
     "sm_surrogate_set_params": [_P, _P],
-    "sm_surrogate_samples": [C.c_uint64, C.c_int64, _I, _P, _P, _P],
+    "sm_surrogate_samples": [C.c_uint64, C.c_int64, _I, _I, _P, _P, _P],
     "sm_surrogate_gradients": [_P, _I, _P, _P, C.POINTER(_D), _P],
     "sm_surrogate_train_step": [_P, _I, _P, _P, C.c_float, C.POINTER(_D)],
     "sm_surrogate_forward": [_P, _I, _P, _I, _P],

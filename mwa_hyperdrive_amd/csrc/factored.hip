@@ -15,14 +15,16 @@
 //     Jones entries. Per step of 16 components it forms A = K C^e s (s a power-of-two scale
 //     keeping |A| inside the fp16 range) and the K operand, splits each value x into
 //     hi = rn(x), lo = rn(x - hi) in the MFMA input type (fp16 or bf16) in LDS, and runs
-//     hi*hi, hi*lo, lo*hi and lo*lo on v_mfma_f32_32x32x16_{f16,bf16} (2 x 2 output tiles of
-//     32 x 32 per wave): hi*hi into one FP32 accumulator, the three corrections into a
-//     second, so their roundings stay apart from the large partial sums. Every `seg_steps`
+//     hi*hi, hi*lo and lo*hi on v_mfma_f32_32x32x16_{f16,bf16} (2 x 2 output tiles of 32 x 32
+//     per wave): hi*hi into one FP32 accumulator, the two corrections into a second, so their
+//     roundings stay apart from the large partial sums. For fp16, lo*lo (about 2^-22 of the
+//     product) is below the other errors and is not computed; bf16 adds it. Every `seg_steps`
 //     steps the FP32 partials are added into an FP64 buffer. Two work-groups share a CU, so
 //     one group's operand generation overlaps the other's MFMAs.
 // A scatter kernel maps the FP64 block results to baselines and accumulates visibilities.
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "sm_factored.h"
@@ -248,7 +250,8 @@ __global__ __launch_bounds__(256, 2) void k_factored(const FactoredArgs a) {
                     acc[rt][ct] = Mfma<E>::run(ahi[rt], bhi[ct], acc[rt][ct]);
                     cor[rt][ct] = Mfma<E>::run(ahi[rt], blo[ct], cor[rt][ct]);
                     cor[rt][ct] = Mfma<E>::run(alo[rt], bhi[ct], cor[rt][ct]);
-                    cor[rt][ct] = Mfma<E>::run(alo[rt], blo[ct], cor[rt][ct]);
+                    // bf16 halves carry 8 bits each, so lo*lo (2^-16 of the product) still counts.
+                    if (std::is_same<E, __bf16>::value) cor[rt][ct] = Mfma<E>::run(alo[rt], blo[ct], cor[rt][ct]);
                 }
         }
         if ((step + 1) % a.seg_steps == 0 || step + 1 == nsteps) {

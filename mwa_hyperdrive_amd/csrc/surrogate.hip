@@ -280,23 +280,41 @@ __device__ void analytic_jones(double az, double za, double freq, const unsigned
 }
 
 // Sample i (global index first + i): az ~ U[0, 2 pi), cos za ~ U[0, 1], f ~ U[167, 197] MHz,
-// delays ~ U{0..31}, each of the 32 gains 0 with probability 0.02 (dead dipole), else 1.
-__device__ void draw(unsigned long long seed, long long gi, double &az, double &za, double &freq, unsigned *delays,
-                     double *gains) {
+// each of the 32 gains 0 with probability 0.02 (dead dipole), else 1, and delays
+//   mode 0: each ~ U{0..31} independently;
+//   mode 1: the delays that point the tile at a random direction within 60 deg of zenith
+//           (cos za0 ~ U[cos 60 deg, 1], az0 ~ U[0, 2 pi)): round((x_d sin za0 sin az0 +
+//           y_d sin za0 cos az0) / (c 435 ps)), shifted so the smallest is 0, capped at 31.
+__device__ void draw(unsigned long long seed, long long gi, int mode, double &az, double &za, double &freq,
+                     unsigned *delays, double *gains) {
     unsigned long long s = seed ^ (0xD1B54A32D192ED03ull * (unsigned long long)(gi + 1));
     az = 6.283185307179586 * uniform(s);
     za = acos(uniform(s));
     freq = 167e6 + 30e6 * uniform(s);
-    for (int d = 0; d < 16; ++d) delays[d] = (unsigned)(uniform(s) * 32.0) & 31u;
+    if (mode == 0) {
+        for (int d = 0; d < 16; ++d) delays[d] = (unsigned)(uniform(s) * 32.0) & 31u;
+    } else {
+        const double az0 = 6.283185307179586 * uniform(s);
+        const double za0 = acos(0.5 + 0.5 * uniform(s));
+        const double se = sin(za0) * sin(az0), sn = sin(za0) * cos(az0);
+        double t[16], lo = 1e30;
+        for (int d = 0; d < 16; ++d) {
+            const double x = ((double)(d % 4) - 1.5) * 1.1, y = (1.5 - (double)(d / 4)) * 1.1;
+            t[d] = rint((x * se + y * sn) / SM_VEL_C / 435e-12);
+            lo = fmin(lo, t[d]);
+        }
+        for (int d = 0; d < 16; ++d) delays[d] = (unsigned)fmin(t[d] - lo, 31.0);
+    }
     for (int d = 0; d < 32; ++d) gains[d] = uniform(s) < 0.02 ? 0.0 : 1.0;
 }
 
-__global__ void k_samples(unsigned long long seed, long long first, int n, float *x, float *y, double *raw) {
+__global__ void k_samples(unsigned long long seed, long long first, int n, int mode, float *x, float *y,
+                          double *raw) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double az, za, freq, gains[32];
     unsigned delays[16];
-    draw(seed, first + i, az, za, freq, delays, gains);
+    draw(seed, first + i, mode, az, za, freq, delays, gains);
     if (x != nullptr) features(az, za, freq, delays, gains, x + (size_t)i * kNIn);
     double j[8];
     analytic_jones(az, za, freq, delays, gains, j);
@@ -553,11 +571,13 @@ const char *sm_surrogate_set_params(SmSurrogate *s, const float *host) {
     return nullptr;
 }
 
-const char *sm_surrogate_samples(uint64_t seed, int64_t first, int32_t n, float *d_x, float *d_y, double *d_raw) {
+const char *sm_surrogate_samples(uint64_t seed, int64_t first, int32_t n, int32_t mode, float *d_x, float *d_y,
+                                 double *d_raw) {
     if (n < 0) return "negative sample count";
+    if (mode != 0 && mode != 1) return "mode must be 0 (independent delays) or 1 (pointing delays)";
     if (n == 0) return nullptr;
     hipLaunchKernelGGL(k_samples, dim3(blocks(n)), dim3(256), 0, 0, (unsigned long long)seed, (long long)first, n,
-                       d_x, d_y, d_raw);
+                       mode, d_x, d_y, d_raw);
     SM_TRY(hipGetLastError());
     return nullptr;
 }

@@ -20,16 +20,21 @@ def _ptr(t) -> C.c_void_p:
     return C.c_void_p(t.data_ptr()) if t is not None else C.c_void_p(None)
 
 
-def samples(seed: int, first: int, n: int, want_raw: bool = False):
+POINTING, INDEPENDENT = 1, 0
+
+
+def samples(seed: int, first: int, n: int, want_raw: bool = False, mode: int = POINTING):
     """Seeded training samples from the exact (analytic) beam, as device tensors.
 
+    mode POINTING: tile delays that steer the beam within 60 deg of zenith (the delays MWA
+    tiles use); INDEPENDENT: each dipole delay uniform in 0..31.
     Returns (x [n][53] f32, y [n][8] f32) or, with want_raw, also raw [n][51] f64."""
     import torch
 
     x = torch.empty((n, N_IN), dtype=torch.float32, device="cuda")
     y = torch.empty((n, N_OUT), dtype=torch.float32, device="cuda")
     raw = torch.empty((n, 51), dtype=torch.float64, device="cuda") if want_raw else None
-    _lib.call("sm_surrogate_samples", C.c_uint64(seed), C.c_int64(first), n, _ptr(x), _ptr(y), _ptr(raw))
+    _lib.call("sm_surrogate_samples", C.c_uint64(seed), C.c_int64(first), n, mode, _ptr(x), _ptr(y), _ptr(raw))
     return (x, y, raw) if want_raw else (x, y)
 
 
@@ -82,7 +87,7 @@ class BeamSurrogate:
         return y
 
     def train(self, n_samples: int, epochs: int, batch: int = 8192, lr: float = 2e-3, seed: int = 7,
-              lr_final: float = 1e-5, log=None) -> dict:
+              lr_final: float = 1e-5, log=None, mode: int = POINTING) -> dict:
         """Adam over `epochs` passes of a fixed seeded sample set, cosine-decayed learning rate.
 
         The samples are generated once on the device (n_samples x 61 floats)."""
@@ -90,7 +95,7 @@ class BeamSurrogate:
 
         import torch
 
-        x, y = samples(seed, 0, n_samples)
+        x, y = samples(seed, 0, n_samples, mode=mode)
         steps_per_epoch = max(1, n_samples // batch)
         total = steps_per_epoch * epochs
         t0 = time.perf_counter()

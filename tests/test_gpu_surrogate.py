@@ -15,8 +15,9 @@ from oracle import surrogate_oracle as so
 pytestmark = pytest.mark.gpu
 
 
-def test_samples_match_oracle_beam_and_features():
-    x, y, raw = sg.samples(11, 1000, 300, want_raw=True)
+@pytest.mark.parametrize("mode", [sg.POINTING, sg.INDEPENDENT])
+def test_samples_match_oracle_beam_and_features(mode):
+    x, y, raw = sg.samples(11, 1000, 300, want_raw=True, mode=mode)
     raw = raw.cpu().numpy()
     np.testing.assert_allclose(x.cpu().numpy(), so.features(raw), rtol=1e-6, atol=1e-7)
     exact = np.array([orc.analytic_beam(r[0], r[1], r[2], r[3:19].astype(np.uint32), r[19:51]) for r in raw])
@@ -24,9 +25,20 @@ def test_samples_match_oracle_beam_and_features():
     # The distribution the generator promises.
     assert (raw[:, 1] >= 0).all() and (raw[:, 1] <= np.pi / 2).all()
     assert (raw[:, 2] >= 167e6).all() and (raw[:, 2] <= 197e6).all()
-    assert set(np.unique(raw[:, 3:19])) <= set(range(32))
+    delays = raw[:, 3:19]
+    assert set(np.unique(delays)) <= set(range(32))
+    if mode == sg.POINTING:
+        # Pointing delays: smallest 0, linear in the dipole position to within rounding.
+        assert (delays.min(axis=1) == 0).all()
+        d = np.arange(16)
+        xy = np.stack([(d % 4 - 1.5) * 1.1, (1.5 - d // 4) * 1.1, np.ones(16)], 1)
+        for row in delays[:50]:
+            if row.max() >= 31:
+                continue
+            coef, *_ = np.linalg.lstsq(xy, row, rcond=None)
+            assert np.abs(xy @ coef - row).max() <= 1.0
     # Same (seed, index) -> same sample, whatever the batch it is drawn in.
-    x2, _ = sg.samples(11, 1100, 5)
+    x2, _ = sg.samples(11, 1100, 5, mode=mode)
     np.testing.assert_array_equal(x2.cpu().numpy(), x.cpu().numpy()[100:105])
 
 
@@ -78,11 +90,13 @@ def test_training_reduces_error_and_fallback_is_exact():
         d = s.forward(xh) - yh
         return d.pow(2).mean().sqrt().item(), d.abs().max().item()
     rms0, max0 = err()
-    info = s.train(n_samples=1 << 17, epochs=8, batch=4096, lr=2e-3)
+    target_rms = yh.pow(2).mean().sqrt().item()
+    info = s.train(n_samples=1 << 19, epochs=12, batch=2048, lr=2e-3)
     rms1, after = err()
-    print(f"held-out |J err|: untrained rms {rms0:.3e} max {max0:.3e} -> trained rms {rms1:.3e} max {after:.3e} "
-          f"({info['steps']} steps, {info['seconds']:.1f} s)")
-    assert rms1 < rms0 / 5
+    print(f"held-out |J err| (pointing delays): target rms {target_rms:.3e}; untrained rms {rms0:.3e} "
+          f"max {max0:.3e} -> trained rms {rms1:.3e} max {after:.3e} ({info['steps']} steps, "
+          f"{info['seconds']:.1f} s)")
+    assert rms1 < 0.7 * min(rms0, target_rms)
     # Fallback: threshold 0 sends every tile to the exact beam; a huge threshold keeps all.
     rng = np.random.default_rng(2)
     delays = np.zeros((3, 16), np.uint32)

@@ -169,7 +169,7 @@ def surrogate_bench(obs) -> dict:
     from mwa_hyperdrive_amd.modeller import analytic_beam_device
 
     s = sg.BeamSurrogate(seed=1)
-    tr = s.train(n_samples=1 << 20, epochs=4, batch=8192, lr=2e-3)
+    tr = s.train(n_samples=1 << 21, epochs=12, batch=4096, lr=2e-3)
     xh, yh = sg.samples(99, 0, 100000)
     d = s.forward(xh) - yh
     rms, mx = d.pow(2).mean().sqrt().item(), d.abs().max().item()
@@ -198,7 +198,8 @@ def surrogate_bench(obs) -> dict:
     err_c2 = (j - exact).abs().max().item()
     res = {
         "network": "53 -> 4 x 256 SiLU -> 8, FP32 training / FP16 MFMA inference",
-        "train": {"samples": 1 << 20, "epochs": 4, "batch": 8192, "steps": tr["steps"], "seconds": tr["seconds"]},
+        "train": {"samples": 1 << 21, "epochs": 12, "batch": 4096, "delays": "pointing (mode 1)", "steps": tr["steps"],
+                  "seconds": tr["seconds"]},
         "heldout_jones_err": {"rms": rms, "max": mx, "samples": 100000},
         "c2_beam": {"evaluations": evals, "max_jones_err": err_c2,
                     "exact_ms": t_exact * 1e3, "surrogate_ms": t_sur * 1e3,

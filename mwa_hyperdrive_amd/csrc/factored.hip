@@ -39,7 +39,12 @@ typedef float f16v __attribute__((ext_vector_type(16)));
 constexpr int kFB = 64;             // tiles per block side
 constexpr int kFC = 16;             // components per step
 constexpr int kFK = 2 * kFC;        // real K per step
-constexpr int kStride = kFK + 8;    // LDS row stride in elements (16-byte aligned, staggered banks)
+// LDS operand rows are kFK = 32 elements (64 bytes, four 16-byte chunks) with no padding;
+// chunk c of row r lives at chunk c ^ swz(r). A 16-byte access by 16 lanes then touches 16
+// distinct 4-bank groups both for the fragment reads (16 rows, one chunk) and for the
+// generation writes (4 rows x 4 chunks).
+__device__ __forceinline__ int swz(int row) { return (row >> 2) & 3; }
+__device__ __forceinline__ int lds_at(int row, int k) { return row * kFK + ((((k >> 3) ^ swz(row)) << 3) | (k & 7)); }
 constexpr size_t kKbufCap = size_t(4) << 30;  // phasor buffer bytes per chunk
 
 template <class E> struct Mfma;
@@ -125,10 +130,10 @@ struct Ops {
 template <class E>
 __global__ __launch_bounds__(256, 2) void k_factored(const FactoredArgs a) {
     typedef typename Mfma<E>::vec V;
-    __shared__ __attribute__((aligned(16))) E s_ahi[2][kFB][kStride];
-    __shared__ __attribute__((aligned(16))) E s_alo[2][kFB][kStride];
-    __shared__ __attribute__((aligned(16))) E s_bhi[2 * kFB][kStride];
-    __shared__ __attribute__((aligned(16))) E s_blo[2 * kFB][kStride];
+    __shared__ __attribute__((aligned(16))) E s_ahi[2][kFB * kFK];
+    __shared__ __attribute__((aligned(16))) E s_alo[2][kFB * kFK];
+    __shared__ __attribute__((aligned(16))) E s_bhi[2 * kFB * kFK];
+    __shared__ __attribute__((aligned(16))) E s_blo[2 * kFB * kFK];
 
     const int total = a.nchan * a.nblocks * 2;
     int b = blockIdx.x;
@@ -203,8 +208,8 @@ __global__ __launch_bounds__(256, 2) void k_factored(const FactoredArgs a) {
                     l[2 * j] = l0;
                     l[2 * j + 1] = l1;
                 }
-                *reinterpret_cast<V *>(&s_ahi[e][gr][2 * gq]) = h;
-                *reinterpret_cast<V *>(&s_alo[e][gr][2 * gq]) = l;
+                *reinterpret_cast<V *>(&s_ahi[e][lds_at(gr, 2 * gq)]) = h;
+                *reinterpret_cast<V *>(&s_alo[e][lds_at(gr, 2 * gq)]) = l;
             }
             // Column gr carries K_q: rows (2c, 2c+1) hold (K_re, K_im) for V_re and
             // (-K_im, K_re) for V_im.
@@ -225,10 +230,10 @@ __global__ __launch_bounds__(256, 2) void k_factored(const FactoredArgs a) {
                 l1[2 * j] = -il;
                 l1[2 * j + 1] = rl;
             }
-            *reinterpret_cast<V *>(&s_bhi[gr][2 * gq]) = h0;
-            *reinterpret_cast<V *>(&s_blo[gr][2 * gq]) = l0;
-            *reinterpret_cast<V *>(&s_bhi[kFB + gr][2 * gq]) = h1;
-            *reinterpret_cast<V *>(&s_blo[kFB + gr][2 * gq]) = l1;
+            *reinterpret_cast<V *>(&s_bhi[lds_at(gr, 2 * gq)]) = h0;
+            *reinterpret_cast<V *>(&s_blo[lds_at(gr, 2 * gq)]) = l0;
+            *reinterpret_cast<V *>(&s_bhi[lds_at(kFB + gr, 2 * gq)]) = h1;
+            *reinterpret_cast<V *>(&s_blo[lds_at(kFB + gr, 2 * gq)]) = l1;
         }
         __syncthreads();
         load(g, step + 2 < nsteps ? step + 2 : nsteps - 1);  // unconditional: no register moves
@@ -237,10 +242,12 @@ __global__ __launch_bounds__(256, 2) void k_factored(const FactoredArgs a) {
             V ahi[2], alo[2], bhi[2], blo[2];
 #pragma unroll
             for (int t2 = 0; t2 < 2; ++t2) {
-                ahi[t2] = *reinterpret_cast<const V *>(&s_ahi[le][t2 * 32 + row][kb * 16 + kofs]);
-                alo[t2] = *reinterpret_cast<const V *>(&s_alo[le][t2 * 32 + row][kb * 16 + kofs]);
-                bhi[t2] = *reinterpret_cast<const V *>(&s_bhi[colbase + t2 * 32 + row][kb * 16 + kofs]);
-                blo[t2] = *reinterpret_cast<const V *>(&s_blo[colbase + t2 * 32 + row][kb * 16 + kofs]);
+                const int ia = lds_at(t2 * 32 + row, kb * 16 + kofs);
+                const int ib = lds_at(colbase + t2 * 32 + row, kb * 16 + kofs);
+                ahi[t2] = *reinterpret_cast<const V *>(&s_ahi[le][ia]);
+                alo[t2] = *reinterpret_cast<const V *>(&s_alo[le][ia]);
+                bhi[t2] = *reinterpret_cast<const V *>(&s_bhi[ib]);
+                blo[t2] = *reinterpret_cast<const V *>(&s_blo[ib]);
             }
 #pragma unroll
             for (int rt = 0; rt < 2; ++rt)

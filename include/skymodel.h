@@ -138,7 +138,7 @@ typedef struct SmInfo {
  * but point components go through the antenna-factorised MFMA path: per channel
  * V^e = (K (.) C^e) K^H with K_pc = exp(i 2 pi (u_p l + v_p m + w_p (n - 1)) f / c), phases and
  * their range reduction in FP64, operands split into hi + lo halves of `dtype`, products on
- * MFMA with FP32 accumulation promoted to FP64 every 512 components. Gaussians and
+ * MFMA with FP32 accumulation promoted to FP64 every 1024 components. Gaussians and
  * shapelets take the FP64 direct kernels. Requires one beam shared by every tile (or no
  * beam); with per-tile beams the points fall back to the FP64 direct kernel. */
 enum { SM_MFMA_FP16 = 0, SM_MFMA_BF16 = 1 };
@@ -146,7 +146,7 @@ const char *sm_model_timestep_factored(SmModeller *m, double lst, double latitud
                                        JonesF32 *d_vis_fb, JonesF64 *d_vis_fb_f64);
 
 /* FP32 accumulation span of the split-precision kernel, in steps of 16 components (default
- * 32 = 512 components): after each span the partial sums are added into FP64. Process-wide. */
+ * 64 = 1024 components): after each span the partial sums are added into FP64. Process-wide. */
 const char *sm_set_factored_segment(int32_t steps);
 
 /* Self-test of the MFMA lane layout the factored kernel assumes: D (32 x 32 f32, row-major)

@@ -123,35 +123,80 @@ struct Ops {
     float4 p[2], q[2], c[4];
 };
 
+// LDS operands of one step: A (hi, lo) for the two Jones entries, B (hi, lo) with the V_re
+// columns 0..63 and V_im columns 64..127. Two such buffers alternate between steps.
+template <class E> struct StepLds {
+    E a[2][2][kFB * kFK];   // [hi / lo][entry]
+    E b[2][2 * kFB * kFK];  // [hi / lo]
+};
+
+// Operand generation for one step from registers into one LDS buffer. Part 0 writes entry 0
+// of A, part 1 entry 1 of A and the B columns, so each half can ride along one K slice of
+// MFMAs. Thread t handles tile t / 4 of each block side and components 4 (t % 4) .. + 3.
+template <class E>
+__device__ __forceinline__ void generate(const Ops &g, StepLds<E> &L, int part, int gr, int gq) {
+    typedef typename Mfma<E>::vec V;
+    const float kr[4] = {g.p[0].x, g.p[0].z, g.p[1].x, g.p[1].z};
+    const float ki[4] = {g.p[0].y, g.p[0].w, g.p[1].y, g.p[1].w};
+    {
+        const int e = part;
+        V h, l;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float cr = e ? g.c[j].z : g.c[j].x;
+            const float ci = e ? g.c[j].w : g.c[j].y;
+            E h0, l0, h1, l1;
+            split<E>(kr[j] * cr - ki[j] * ci, h0, l0);
+            split<E>(kr[j] * ci + ki[j] * cr, h1, l1);
+            h[2 * j] = h0;
+            h[2 * j + 1] = h1;
+            l[2 * j] = l0;
+            l[2 * j + 1] = l1;
+        }
+        *reinterpret_cast<V *>(&L.a[0][e][lds_at(gr, 2 * gq)]) = h;
+        *reinterpret_cast<V *>(&L.a[1][e][lds_at(gr, 2 * gq)]) = l;
+    }
+    if (part == 0) return;
+    // Column gr carries K_q: rows (2c, 2c+1) hold (K_re, K_im) for V_re and (-K_im, K_re)
+    // for V_im.
+    const float qr[4] = {g.q[0].x, g.q[0].z, g.q[1].x, g.q[1].z};
+    const float qi[4] = {g.q[0].y, g.q[0].w, g.q[1].y, g.q[1].w};
+    V h0, l0, h1, l1;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        E rh, rl, ih, il;
+        split<E>(qr[j], rh, rl);
+        split<E>(qi[j], ih, il);
+        h0[2 * j] = rh;
+        h0[2 * j + 1] = ih;
+        l0[2 * j] = rl;
+        l0[2 * j + 1] = il;
+        h1[2 * j] = -ih;
+        h1[2 * j + 1] = rh;
+        l1[2 * j] = -il;
+        l1[2 * j + 1] = rl;
+    }
+    *reinterpret_cast<V *>(&L.b[0][lds_at(gr, 2 * gq)]) = h0;
+    *reinterpret_cast<V *>(&L.b[1][lds_at(gr, 2 * gq)]) = l0;
+    *reinterpret_cast<V *>(&L.b[0][lds_at(kFB + gr, 2 * gq)]) = h1;
+    *reinterpret_cast<V *>(&L.b[1][lds_at(kFB + gr, 2 * gq)]) = l1;
+}
+
 // Four waves: wave w computes local Jones entry w % 2 (entry 2 * half + w % 2) for output
 // columns (w / 2) * 64 .. + 63 (V_re columns, then V_im columns), 2 x 2 tiles of 32 x 32.
-// Generation: thread t handles tile t / 4 of each block side and components 4 (t % 4) ..
-// 4 (t % 4) + 3 of the step, one 16-byte LDS row segment per operand.
-template <class E>
-__global__ __launch_bounds__(256, 2) void k_factored(const FactoredArgs a) {
+// Each step s runs the MFMAs of step s from one LDS buffer while generating step s + 1 into
+// the other, in one basic block with the MFMAs and the generation's ALU work interleaved,
+// and a single barrier per step. DIAG: a diagonal block, whose output tile (rows 32..63,
+// columns 0..31 of each half) holds only pairs p > q that no baseline reads; it is skipped.
+template <class E, bool DIAG>
+__device__ __forceinline__ void factored_block(const FactoredArgs &a, StepLds<E> (&buf)[2], int f, int blk,
+                                               int half, int2 pq) {
     typedef typename Mfma<E>::vec V;
-    __shared__ __attribute__((aligned(16))) E s_ahi[2][kFB * kFK];
-    __shared__ __attribute__((aligned(16))) E s_alo[2][kFB * kFK];
-    __shared__ __attribute__((aligned(16))) E s_bhi[2 * kFB * kFK];
-    __shared__ __attribute__((aligned(16))) E s_blo[2 * kFB * kFK];
-
-    const int total = a.nchan * a.nblocks * 2;
-    int b = blockIdx.x;
-    if ((total & 7) == 0) b = (b & 7) * (total >> 3) + (b >> 3);
-    const int f = b / (2 * a.nblocks);
-    const int rem = b - f * 2 * a.nblocks;
-    const int blk = rem >> 1;
-    const int half = rem & 1;
-    const int2 pq = a.blocks[blk];
-    // In a diagonal block the output tile (rows 32..63, columns 0..31 of each half) holds only
-    // pairs p > q, which no baseline reads: its MFMAs and its flush are skipped.
-    const bool diag = pq.x == pq.y;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int le = wave & 1;
     const int colbase = (wave >> 1) * kFB;
-
     const int gr = tid >> 2;
     const int gq = (tid & 3) * 4;
     // Work-group-uniform bases with 32-bit per-thread offsets keep the addresses out of the
@@ -159,7 +204,6 @@ __global__ __launch_bounds__(256, 2) void k_factored(const FactoredArgs a) {
     const float2 *kf = a.kbuf + (size_t)f * a.tpad * a.cpad;
     const unsigned offp = (unsigned)((pq.x * kFB + gr) * a.cpad + gq);
     const unsigned offq = (unsigned)((pq.y * kFB + gr) * a.cpad + gq);
-
     // Scaled FP32 coherencies of the thread's 4 components: (re, im) of the work-group's first
     // Jones entry, then of its second, one float4 per component.
     const float4 *cf = a.cohf + (size_t)f * a.cpad * 2 + gq * 2 + half;
@@ -184,83 +228,61 @@ __global__ __launch_bounds__(256, 2) void k_factored(const FactoredArgs a) {
     const int nsteps = a.cpad / kFC;
     const int row = lane & 31;
     const int kofs = (lane >> 5) * 8;
+    constexpr bool kBfLoLo = std::is_same<E, __bf16>::value;
 
-    // One step: operands from registers into LDS, then the MFMAs. The registers are
-    // reloaded with the step two ahead as soon as they are in LDS, so every global load has
-    // two steps of MFMAs to arrive; two register sets alternate.
-    auto run_step = [&](Ops &g, int step) {
+    Ops g;
+    load(g, 0);
+    generate<E>(g, buf[0], 0, gr, gq);
+    generate<E>(g, buf[0], 1, gr, gq);
+    load(g, nsteps > 1 ? 1 : 0);
+    for (int step = 0; step < nsteps; ++step) {
+        const int cur = step & 1;
         __syncthreads();
-        {
-            const float kr[4] = {g.p[0].x, g.p[0].z, g.p[1].x, g.p[1].z};
-            const float ki[4] = {g.p[0].y, g.p[0].w, g.p[1].y, g.p[1].w};
-#pragma unroll
-            for (int e = 0; e < 2; ++e) {
-                V h, l;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const float cr = e ? g.c[j].z : g.c[j].x;
-                    const float ci = e ? g.c[j].w : g.c[j].y;
-                    E h0, l0, h1, l1;
-                    split<E>(kr[j] * cr - ki[j] * ci, h0, l0);
-                    split<E>(kr[j] * ci + ki[j] * cr, h1, l1);
-                    h[2 * j] = h0;
-                    h[2 * j + 1] = h1;
-                    l[2 * j] = l0;
-                    l[2 * j + 1] = l1;
-                }
-                *reinterpret_cast<V *>(&s_ahi[e][lds_at(gr, 2 * gq)]) = h;
-                *reinterpret_cast<V *>(&s_alo[e][lds_at(gr, 2 * gq)]) = l;
-            }
-            // Column gr carries K_q: rows (2c, 2c+1) hold (K_re, K_im) for V_re and
-            // (-K_im, K_re) for V_im.
-            const float qr[4] = {g.q[0].x, g.q[0].z, g.q[1].x, g.q[1].z};
-            const float qi[4] = {g.q[0].y, g.q[0].w, g.q[1].y, g.q[1].w};
-            V h0, l0, h1, l1;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                E rh, rl, ih, il;
-                split<E>(qr[j], rh, rl);
-                split<E>(qi[j], ih, il);
-                h0[2 * j] = rh;
-                h0[2 * j + 1] = ih;
-                l0[2 * j] = rl;
-                l0[2 * j + 1] = il;
-                h1[2 * j] = -ih;
-                h1[2 * j + 1] = rh;
-                l1[2 * j] = -il;
-                l1[2 * j + 1] = rl;
-            }
-            *reinterpret_cast<V *>(&s_bhi[lds_at(gr, 2 * gq)]) = h0;
-            *reinterpret_cast<V *>(&s_blo[lds_at(gr, 2 * gq)]) = l0;
-            *reinterpret_cast<V *>(&s_bhi[lds_at(kFB + gr, 2 * gq)]) = h1;
-            *reinterpret_cast<V *>(&s_blo[lds_at(kFB + gr, 2 * gq)]) = l1;
-        }
-        __syncthreads();
-        load(g, step + 2 < nsteps ? step + 2 : nsteps - 1);  // unconditional: no register moves
-#pragma unroll 1
-        for (int kb = 0; kb < kFK / 16; ++kb) {
+        auto kslice = [&](auto kbc) {
+            constexpr int kb = decltype(kbc)::value;
             V ahi[2], alo[2], bhi[2], blo[2];
 #pragma unroll
             for (int t2 = 0; t2 < 2; ++t2) {
                 const int ia = lds_at(t2 * 32 + row, kb * 16 + kofs);
                 const int ib = lds_at(colbase + t2 * 32 + row, kb * 16 + kofs);
-                ahi[t2] = *reinterpret_cast<const V *>(&s_ahi[le][ia]);
-                alo[t2] = *reinterpret_cast<const V *>(&s_alo[le][ia]);
-                bhi[t2] = *reinterpret_cast<const V *>(&s_bhi[ib]);
-                blo[t2] = *reinterpret_cast<const V *>(&s_blo[ib]);
+                ahi[t2] = *reinterpret_cast<const V *>(&buf[cur].a[0][le][ia]);
+                alo[t2] = *reinterpret_cast<const V *>(&buf[cur].a[1][le][ia]);
+                bhi[t2] = *reinterpret_cast<const V *>(&buf[cur].b[0][ib]);
+                blo[t2] = *reinterpret_cast<const V *>(&buf[cur].b[1][ib]);
             }
+            // Product rounds over the tiles, so that MFMAs writing the same accumulator are
+            // never back to back: hi*hi, hi*lo, lo*hi (and, for bf16, whose halves carry 8 bits
+            // each, lo*lo at 2^-16 of the product).
 #pragma unroll
-            for (int rt = 0; rt < 2; ++rt)
+            for (int round = 0; round < (kBfLoLo ? 4 : 3); ++round)
 #pragma unroll
-                for (int ct = 0; ct < 2; ++ct) {
-                    if (diag && rt == 1 && ct == 0) continue;
-                    acc[rt][ct] = Mfma<E>::run(ahi[rt], bhi[ct], acc[rt][ct]);
-                    cor[rt][ct] = Mfma<E>::run(ahi[rt], blo[ct], cor[rt][ct]);
-                    cor[rt][ct] = Mfma<E>::run(alo[rt], bhi[ct], cor[rt][ct]);
-                    // bf16 halves carry 8 bits each, so lo*lo (2^-16 of the product) still counts.
-                    if (std::is_same<E, __bf16>::value) cor[rt][ct] = Mfma<E>::run(alo[rt], blo[ct], cor[rt][ct]);
-                }
-        }
+                for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+                    for (int ct = 0; ct < 2; ++ct) {
+                        if (DIAG && rt == 1 && ct == 0) continue;
+                        if (round == 0) acc[rt][ct] = Mfma<E>::run(ahi[rt], bhi[ct], acc[rt][ct]);
+                        if (round == 1) cor[rt][ct] = Mfma<E>::run(ahi[rt], blo[ct], cor[rt][ct]);
+                        if (round == 2) cor[rt][ct] = Mfma<E>::run(alo[rt], bhi[ct], cor[rt][ct]);
+                        if (round == 3) cor[rt][ct] = Mfma<E>::run(alo[rt], blo[ct], cor[rt][ct]);
+                    }
+            // The next step's operands into the other buffer while these MFMAs run (after the
+            // last step this fills a buffer nobody reads; no branch splits the block).
+            generate<E>(g, buf[cur ^ 1], kb, gr, gq);
+            // Issue order: the fragment reads, then each MFMA followed by a share of the
+            // generation's ALU work, then the generation's LDS writes.
+            constexpr int kM = (DIAG ? 3 : 4) * (kBfLoLo ? 4 : 3);
+            constexpr int kV = kb == 0 ? 4 : 8;
+            __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);
+#pragma unroll
+            for (int i = 0; i < kM; ++i) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x002, kV, 0);
+            }
+            __builtin_amdgcn_sched_group_barrier(0x200, kb == 0 ? 2 : 6, 0);
+        };
+        kslice(std::integral_constant<int, 0>{});
+        kslice(std::integral_constant<int, 1>{});
+        load(g, step + 2 < nsteps ? step + 2 : nsteps - 1);
         if ((step + 1) % a.seg_steps == 0 || step + 1 == nsteps) {
             // Output layout of v_mfma_f32_32x32x16: element r of lane l is row
             // 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32. The work-group is the only
@@ -274,7 +296,7 @@ __global__ __launch_bounds__(256, 2) void k_factored(const FactoredArgs a) {
             for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
                 for (int ct = 0; ct < 2; ++ct) {
-                    if (diag && rt == 1 && ct == 0) continue;
+                    if (DIAG && rt == 1 && ct == 0) continue;
 #pragma unroll
                     for (int r = 0; r < 16; ++r) {
                         const int o = (rt * 32 + 8 * (r >> 2) + (r & 3)) * (2 * kFB) + ct * 32;
@@ -285,15 +307,24 @@ __global__ __launch_bounds__(256, 2) void k_factored(const FactoredArgs a) {
                     }
                 }
         }
-    };
-
-    Ops g0, g1;
-    load(g0, 0);
-    load(g1, nsteps > 1 ? 1 : 0);
-    for (int step = 0; step < nsteps; step += 2) {
-        run_step(g0, step);
-        if (step + 1 < nsteps) run_step(g1, step + 1);
     }
+}
+
+template <class E>
+__global__ __launch_bounds__(256, 2) void k_factored(const FactoredArgs a) {
+    __shared__ __attribute__((aligned(16))) StepLds<E> buf[2];
+    const int total = a.nchan * a.nblocks * 2;
+    int b = blockIdx.x;
+    if ((total & 7) == 0) b = (b & 7) * (total >> 3) + (b >> 3);
+    const int f = b / (2 * a.nblocks);
+    const int rem = b - f * 2 * a.nblocks;
+    const int blk = rem >> 1;
+    const int half = rem & 1;
+    const int2 pq = a.blocks[blk];
+    if (pq.x == pq.y)
+        factored_block<E, true>(a, buf, f, blk, half, pq);
+    else
+        factored_block<E, false>(a, buf, f, blk, half, pq);
 }
 
 // FP64 block results -> visibilities [freq][baseline] (+=), undoing the scale. map[i] =

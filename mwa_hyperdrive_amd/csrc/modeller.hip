@@ -618,7 +618,7 @@ extern "C" const char *sm_update_with_a_source(SmModeller *m, const SmComponent 
 }
 
 // Steps of 16 components per FP32 accumulation span of the MFMA point kernel.
-static int g_factored_seg_steps = 32;
+static int g_factored_seg_steps = 64;
 
 extern "C" const char *sm_set_factored_segment(int32_t steps) {
     if (steps < 1 || steps > 4096) return "segment must be 1..4096 steps";

@@ -88,6 +88,19 @@ hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf
     e = launch_coherency(n, ctx.nchan, ctx.nchan_pad, ctx.d_freqs_pad, in.fb, ctx.beam, !per_tile, coh.as<double>(),
                          stream);
     if (e != hipSuccess) return e;
+    // Shapelet coefficients in the kernels' packed form (per thread: handles are not shared).
+    thread_local DevBuf prange, packed;
+    const int4 *d_prange = nullptr;
+    const PackedCoeff *d_packed = nullptr;
+    if (kind == KIND_SHAPELET) {
+        e = packed.reserve(sizeof(PackedCoeff) * (in.coeff_count > 0 ? in.coeff_count : 1));
+        if (e == hipSuccess) e = prange.reserve(sizeof(int4) * n);
+        if (e == hipSuccess)
+            e = launch_pack_coeffs(n, in.coeff_range, in.coeffs, prange.as<int4>(), packed.as<PackedCoeff>(), stream);
+        if (e != hipSuccess) return e;
+        d_prange = prange.as<int4>();
+        d_packed = packed.as<PackedCoeff>();
+    }
     DirectArgs a;
     a.nbl = ctx.nbl;
     a.nchan = ctx.nchan;
@@ -99,8 +112,8 @@ hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf
     a.shape = in.shape;
     a.coh = coh.as<double>();
     a.bl_uv = in.bl_uv;
-    a.coeff_range = in.coeff_range;
-    a.coeffs = in.coeffs;
+    a.prange = d_prange;
+    a.packed = d_packed;
     a.sbf = ctx.d_sbf;
     a.beam = ctx.beam;
     a.vis32 = vis32;
@@ -129,7 +142,7 @@ hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf
         t.vis32 = vis32;
         t.vis64 = vis64;
         e = launch_direct_tiled(kind, t, stream);
-    } else if (g_direct_variant == 0 && kind == KIND_SHAPELET) {
+    } else if (g_direct_variant != 1 && kind == KIND_SHAPELET) {
         ShapeletChanArgs s;
         s.nbl = ctx.nbl;
         s.nchan = ctx.nchan;
@@ -142,8 +155,8 @@ hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf
         s.shape = in.shape;
         s.coh = coh.as<double>();
         s.bl_uv = in.bl_uv;
-        s.coeff_range = in.coeff_range;
-        s.coeffs = in.coeffs;
+        s.prange = d_prange;
+        s.packed = d_packed;
         s.sbf = ctx.d_sbf;
         s.beam = ctx.beam;
         s.vis32 = vis32;
@@ -394,6 +407,7 @@ extern "C" const char *sm_model_shapelets_f64(const Shapelets *s, const Addresse
     in.bl_uv = st.bl_uv.as<double2>();
     in.coeff_range = st.coeff_range.as<int2>();
     in.coeffs = st.coeffs.as<ShapeletCoeff>();
+    in.coeff_count = (int)off;
     SM_TRY(run_type(KIND_SHAPELET, in, ctx, st.coh, d_vis_fb, d_vis_fb_f64, 0, nullptr, nullptr));
     SM_TRY(hipStreamSynchronize(0));
     return nullptr;

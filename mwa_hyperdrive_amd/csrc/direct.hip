@@ -81,14 +81,56 @@ hipError_t launch_coherency(int ncomp, int nchan, int nchan_pad, const double *d
 // basis table (src/model/cpu.rs:633-695, src/gpu/model.cu:39-97). Positions below the
 // table clamp to index 0 with the unclamped fractional weight, as both reference paths do;
 // positions above it clamp to the last interval, whose values are exact zeros.
-__device__ __forceinline__ void shapelet_envelope(double x_pos, double y_pos, int2 range,
-                                                  const ShapeletCoeff *__restrict__ coeffs,
+__device__ __forceinline__ void shapelet_envelope(double x_pos, double y_pos, int4 range,
+                                                  const PackedCoeff *__restrict__ packed,
                                                   const double *__restrict__ sbf, double &er, double &ei) {
     const double fx = floor(x_pos), fy = floor(y_pos);
     const int ix = (int)fmin(fmax(fx, 0.0), (double)(SM_SBF_L - 2));
     const int iy = (int)fmin(fmax(fy, 0.0), (double)(SM_SBF_L - 2));
     const double wx = x_pos - fx, wy = y_pos - fy;
-    shapelet_sum(coeffs + range.x, range.y, sbf, ix, iy, wx, wy, er, ei);
+    shapelet_sum(packed, range, sbf, ix, iy, wx, wy, er, ei);
+}
+
+// One thread per component: its coefficients with i^((n1 + n2) mod 4) resolved. Terms with
+// an even power (real part) first, then odd (imaginary part), each in source order, the sign
+// of -1 folded into the value (exact). Each part of the envelope then adds exactly the terms
+// it added before, in the same order. prange = (offset, real terms, imaginary terms, number
+// of table rows the component uses).
+__global__ void k_pack_coeffs(int ncomp, const int2 *__restrict__ range, const ShapeletCoeff *__restrict__ coeffs,
+                              int4 *prange, PackedCoeff *packed) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncomp) return;
+    const int2 r = range[c];
+    int o = r.x, rows = 0;
+    for (int part = 0; part < 2; ++part) {
+        for (int j = 0; j < r.y; ++j) {
+            const ShapeletCoeff cf = coeffs[r.x + j];
+            const int q = (cf.n1 + cf.n2) & 3;
+            if ((q & 1) != part) continue;
+            const int n1 = cf.n1 < SM_SBF_N ? cf.n1 : SM_SBF_N - 1;
+            const int n2 = cf.n2 < SM_SBF_N ? cf.n2 : SM_SBF_N - 1;
+            PackedCoeff p;
+            p.value = (q & 2) ? -cf.value : cf.value;
+            p.n1 = n1;
+            p.n2 = n2;
+            packed[o++] = p;
+            rows = max(rows, max(n1, n2) + 1);
+        }
+    }
+    const int nre = [&] {
+        int k = 0;
+        for (int j = 0; j < r.y; ++j) k += ((coeffs[r.x + j].n1 + coeffs[r.x + j].n2) & 1) == 0;
+        return k;
+    }();
+    prange[c] = make_int4(r.x, nre, r.y - nre, rows);
+}
+
+hipError_t launch_pack_coeffs(int ncomp, const int2 *d_range, const ShapeletCoeff *d_coeffs, int4 *d_prange,
+                              PackedCoeff *d_packed, hipStream_t stream) {
+    if (ncomp <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pack_coeffs, dim3((ncomp + 255) / 256), dim3(256), 0, stream, ncomp, d_range, d_coeffs,
+                       d_prange, d_packed);
+    return hipGetLastError();
 }
 
 // ------------------------------------------------------------------ direct kernel
@@ -142,7 +184,7 @@ __global__ __launch_bounds__(256) void k_direct(const DirectArgs a, int gx, int 
         }
         double q = 0.0, env = 1.0, er = 1.0, eg = 1.0;
         double xk = 0.0, yk = 0.0;
-        int2 crange = make_int2(0, 0);
+        int4 crange = make_int4(0, 0, 0, 0);
         if (KIND == KIND_GAUSSIAN) {
             const double4 g = shape[c];
             q = g.x * (uvw.u * uvw.u) + g.y * (uvw.u * uvw.v) + g.z * (uvw.v * uvw.v);
@@ -156,7 +198,7 @@ __global__ __launch_bounds__(256) void k_direct(const DirectArgs a, int gx, int 
             const double2 suv = a.bl_uv[(size_t)c * a.nbl + bl];
             xk = (suv.x * sh.x + suv.y * sh.y) * sh.z;
             yk = (suv.x * sh.y - suv.y * sh.x) * sh.w;
-            crange = a.coeff_range[c];
+            crange = a.prange[c];
         }
         const double *cc = coh + ((size_t)c * a.nchan_pad + f0) * 8;
 #pragma unroll
@@ -175,7 +217,7 @@ __global__ __launch_bounds__(256) void k_direct(const DirectArgs a, int gx, int 
             } else if (KIND == KIND_SHAPELET) {
                 const double fk = freqs[f0 + k];
                 double sr, si;
-                shapelet_envelope(xk * fk + SM_SBF_C, yk * fk + SM_SBF_C, crange, a.coeffs, a.sbf, sr, si);
+                shapelet_envelope(xk * fk + SM_SBF_C, yk * fk + SM_SBF_C, crange, a.packed, a.sbf, sr, si);
                 const double tr = pr * sr - pi * si;
                 const double ti = pr * si + pi * sr;
                 pr = tr;

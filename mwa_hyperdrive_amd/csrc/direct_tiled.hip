@@ -22,6 +22,8 @@
 namespace sm {
 
 constexpr int kTG = 16;   // tiles per group
+constexpr int kWin = 32;  // shapelet table window per wave and axis, cells
+constexpr int kRows = 16; // shapelet table rows staged per wave
 constexpr int kCT = 32;   // components per LDS stage
 
 template <int KIND>
@@ -211,8 +213,17 @@ __global__ __launch_bounds__(256) void k_shapelet_chan(const ShapeletChanArgs a)
         rows = a.beam.bl_rows[bl];
         uf = a.beam.freq_map[f];
     }
-    const ShapeletCoeff *__restrict__ coeffs = a.coeffs;
+    const PackedCoeff *__restrict__ packed = a.packed;
     const double *__restrict__ sbf = a.sbf;
+    // Per-wave windows of the basis table (kRows rows x kWin cells per axis).
+    __shared__ double wx_lds[4][kRows][kWin], wy_lds[4][kRows][kWin];
+    const int wv = threadIdx.x >> 6;
+    // The wave's lowest and highest channel frequency (invalid lanes repeat the last channel).
+    double fmin_w = freq, fmax_w = freq;
+    for (int o = 32; o > 0; o >>= 1) {
+        fmin_w = fmin(fmin_w, __shfl_xor(fmin_w, o));
+        fmax_w = fmax(fmax_w, __shfl_xor(fmax_w, o));
+    }
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int c = 0; c < a.ncomp; ++c) {
         const double4 l = a.lmn[c];
@@ -230,8 +241,57 @@ __global__ __launch_bounds__(256) void k_shapelet_chan(const ShapeletChanArgs a)
         const int iy = (int)fmin(fmax(fy, 0.0), (double)(SM_SBF_L - 2));
         const double wx = x_pos - fx, wy = y_pos - fy;
         double er, ei;
-        const int2 range = a.coeff_range[c];
-        shapelet_sum(coeffs + range.x, range.y, sbf, ix, iy, wx, wy, er, ei);
+        const int4 pr4 = a.prange[c];
+        // The cells the wave touches: positions are monotone in frequency, so the extreme
+        // channels bound them (the same expressions, so the same roundings).
+        const double xa = xk * fmin_w + SM_SBF_C, xb = xk * fmax_w + SM_SBF_C;
+        const double ya = yk * fmin_w + SM_SBF_C, yb = yk * fmax_w + SM_SBF_C;
+        const int ix0 = (int)fmin(fmax(floor(fmin(xa, xb)), 0.0), (double)(SM_SBF_L - 2));
+        const int ix1 = (int)fmin(fmax(floor(fmax(xa, xb)), 0.0), (double)(SM_SBF_L - 2));
+        const int iy0 = (int)fmin(fmax(floor(fmin(ya, yb)), 0.0), (double)(SM_SBF_L - 2));
+        const int iy1 = (int)fmin(fmax(floor(fmax(ya, yb)), 0.0), (double)(SM_SBF_L - 2));
+        const int xw = __builtin_amdgcn_readfirstlane(ix1 - ix0 + 2);
+        const int yw = __builtin_amdgcn_readfirstlane(iy1 - iy0 + 2);
+        if (xw <= kWin && yw <= kWin && pr4.w <= kRows) {
+            // Stage rows 0 .. pr4.w - 1 of both windows in this wave's LDS, then interpolate
+            // from LDS: one 16-byte read per axis per coefficient instead of two gathers.
+            const int bx0 = __builtin_amdgcn_readfirstlane(ix0), by0 = __builtin_amdgcn_readfirstlane(iy0);
+            const int j = lane & (kWin - 1), half = lane / kWin;
+            for (int r = half; r < pr4.w; r += 64 / kWin) {
+                if (j < xw) wx_lds[wv][r][j] = sbf[(size_t)r * SM_SBF_L + bx0 + j];
+                if (j < yw) wy_lds[wv][r][j] = sbf[(size_t)r * SM_SBF_L + by0 + j];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const int jx = ix - bx0, jy = iy - by0;
+            double part[2];
+#pragma unroll
+            for (int pt = 0; pt < 2; ++pt) {
+                const PackedCoeff *cp = packed + pr4.x + (pt ? pr4.y : 0);
+                const int cnt = pt ? pr4.z : pr4.y;
+                double accp = 0.0;
+                for (int k = 0; k < cnt; ++k) {
+                    double val;
+                    int n1, n2;
+                    load_packed(cp + k, val, n1, n2);
+                    const double xl = wx_lds[wv][n1][jx], xh = wx_lds[wv][n1][jx + 1];
+                    const double yl = wy_lds[wv][n2][jy], yh = wy_lds[wv][n2][jy + 1];
+                    const double u_value = xl + (xh - xl) * wx;
+                    const double v_value = yl + (yh - yl) * wy;
+                    accp += (val * u_value) * v_value;
+                }
+                part[pt] = accp;
+            }
+            er = part[0];
+            ei = part[1];
+            // Every lane's reads are done before the next component's staging.
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        } else {
+            shapelet_sum(packed, pr4, sbf, ix, iy, wx, wy, er, ei);
+        }
         const double pr = zr * er - zi * ei;
         const double pi = zr * ei + zi * er;
         double cj[8];

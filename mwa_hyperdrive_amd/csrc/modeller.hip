@@ -374,6 +374,7 @@ TypeInputs type_inputs(DevType &d) {
     in.bl_uv = d.bl_uv.as<double2>();
     in.coeff_range = d.ranges.as<int2>();
     in.coeffs = d.coeffs.as<ShapeletCoeff>();
+    in.coeff_count = (int)d.h.coeffs.size();
     return in;
 }
 

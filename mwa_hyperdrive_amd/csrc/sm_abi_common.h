@@ -14,6 +14,7 @@ struct TypeInputs {
     const double2 *bl_uv = nullptr;
     const int2 *coeff_range = nullptr;
     const ShapeletCoeff *coeffs = nullptr;
+    int coeff_count = 0;  // total coefficients behind coeffs
 };
 
 // Per-call modelling context.

@@ -48,6 +48,15 @@ struct BeamDesc {
     const int *tile_rows;   // [ntiles] unique-tile row per unflagged tile (autos)
 };
 
+// A shapelet coefficient as the kernels read it: the value with the sign of
+// i^((n1 + n2) mod 4) folded in, and the basis-table rows n1, n2. Per component the terms
+// that land in the real part of the envelope come first, then those that land in the
+// imaginary part, each group in source order (launch_pack_coeffs).
+struct PackedCoeff {
+    double value;
+    int n1, n2;
+};
+
 struct DirectArgs {
     int nbl, nchan, nchan_pad, ncomp;
     const double *freqs;      // [nchan_pad]
@@ -56,8 +65,8 @@ struct DirectArgs {
     const double4 *shape;     // [ncomp]
     const double *coh;        // [ncomp][nchan_pad][8]
     const double2 *bl_uv;     // [ncomp][nbl]
-    const int2 *coeff_range;  // [ncomp] (offset, count)
-    const ShapeletCoeff *coeffs;
+    const int4 *prange;       // [ncomp] (offset, real-part terms, imaginary-part terms, rows used)
+    const PackedCoeff *packed;
     const double *sbf;        // [SM_SBF_N][SM_SBF_L]
     BeamDesc beam;            // per-tile mode only
     JonesF32 *vis32;          // [nchan][nbl]
@@ -67,6 +76,8 @@ struct DirectArgs {
 };
 
 // Launchers (direct.hip). All enqueue on `stream` and return hipSuccess or an error.
+hipError_t launch_pack_coeffs(int ncomp, const int2 *d_range, const ShapeletCoeff *d_coeffs, int4 *d_prange,
+                              PackedCoeff *d_packed, hipStream_t stream);
 hipError_t launch_coherency(int ncomp, int nchan, int nchan_pad, const double *d_freqs, FluxBuckets fb,
                             BeamDesc beam, bool apply_beam, double *d_coh, hipStream_t stream);
 hipError_t launch_direct(Kind kind, bool per_tile, bool even, const DirectArgs &a, hipStream_t stream);

@@ -5,62 +5,61 @@
 
 namespace sm {
 
-// One shapelet coefficient as a single 16-byte load (a scalar load when the address is
-// wave-uniform): the value in the first 8 bytes, n1 and n2 in the low bytes of the third
-// dword. Byte-wide fields alone would need a vector load and a wait per coefficient.
-__device__ __forceinline__ void load_coeff(const ShapeletCoeff *p, double &value, int &n1, int &n2) {
+// One packed coefficient as a single 16-byte load (a scalar load when the address is
+// wave-uniform).
+__device__ __forceinline__ void load_packed(const PackedCoeff *p, double &value, int &n1, int &n2) {
     const int4 w = *reinterpret_cast<const int4 *>(p);
     value = __hiloint2double(w.y, w.x);
-    n1 = w.z & 0xff;
-    n2 = (w.z >> 8) & 0xff;
+    n1 = w.z;
+    n2 = w.w;
 }
 
-// sum_k i^((n1 + n2) mod 4) c_k B_n1(x) B_n2(y) with B linearly interpolated at cell (ix, iy),
-// fractions (wx, wy) (src/model/cpu.rs:633-695), terms added in coefficient order. Table
-// reads of four coefficients are issued before any of them is used.
-__device__ __forceinline__ void shapelet_sum(const ShapeletCoeff *__restrict__ cp, int count,
-                                             const double *__restrict__ sbf, int ix, int iy, double wx, double wy,
-                                             double &er, double &ei) {
-    er = 0.0;
-    ei = 0.0;
-    auto term = [&](double value, int q, double xl, double xh, double yl, double yh) {
-        const double u_value = xl + (xh - xl) * wx;
-        const double v_value = yl + (yh - yl) * wy;
-        const double t = (value * u_value) * v_value;
-        switch (q) {
-        case 0: er += t; break;
-        case 1: ei += t; break;
-        case 2: er -= t; break;
-        default: ei -= t; break;
-        }
-    };
+// One part (real or imaginary) of sum_k i^((n1 + n2) mod 4) c_k B_n1(x) B_n2(y), with B
+// linearly interpolated at cell (ix, iy), fractions (wx, wy) (src/model/cpu.rs:633-695):
+// the terms of that part in source order, their signs folded into the values. Table reads
+// of four coefficients are issued before any of them is used.
+__device__ __forceinline__ double shapelet_part(const PackedCoeff *__restrict__ cp, int count,
+                                                const double *__restrict__ sbf, int ix, int iy, double wx,
+                                                double wy) {
+    double acc = 0.0;
     int j = 0;
     for (; j + 4 <= count; j += 4) {
         double val[4], xl[4], xh[4], yl[4], yh[4];
-        int q[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             int n1, n2;
-            load_coeff(cp + j + u, val[u], n1, n2);
-            q[u] = (n1 + n2) & 3;
-            const double *r1 = sbf + (size_t)(n1 < SM_SBF_N ? n1 : SM_SBF_N - 1) * SM_SBF_L + ix;
-            const double *r2 = sbf + (size_t)(n2 < SM_SBF_N ? n2 : SM_SBF_N - 1) * SM_SBF_L + iy;
-            xl[u] = r1[0];
-            xh[u] = r1[1];
-            yl[u] = r2[0];
-            yh[u] = r2[1];
+            load_packed(cp + j + u, val[u], n1, n2);
+            const int o1 = n1 * SM_SBF_L, o2 = n2 * SM_SBF_L;
+            xl[u] = sbf[o1 + ix];
+            xh[u] = sbf[o1 + ix + 1];
+            yl[u] = sbf[o2 + iy];
+            yh[u] = sbf[o2 + iy + 1];
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) term(val[u], q[u], xl[u], xh[u], yl[u], yh[u]);
+        for (int u = 0; u < 4; ++u) {
+            const double u_value = xl[u] + (xh[u] - xl[u]) * wx;
+            const double v_value = yl[u] + (yh[u] - yl[u]) * wy;
+            acc += (val[u] * u_value) * v_value;
+        }
     }
     for (; j < count; ++j) {
         double val;
         int n1, n2;
-        load_coeff(cp + j, val, n1, n2);
-        const double *r1 = sbf + (size_t)(n1 < SM_SBF_N ? n1 : SM_SBF_N - 1) * SM_SBF_L + ix;
-        const double *r2 = sbf + (size_t)(n2 < SM_SBF_N ? n2 : SM_SBF_N - 1) * SM_SBF_L + iy;
-        term(val, (n1 + n2) & 3, r1[0], r1[1], r2[0], r2[1]);
+        load_packed(cp + j, val, n1, n2);
+        const int o1 = n1 * SM_SBF_L, o2 = n2 * SM_SBF_L;
+        const double u_value = sbf[o1 + ix] + (sbf[o1 + ix + 1] - sbf[o1 + ix]) * wx;
+        const double v_value = sbf[o2 + iy] + (sbf[o2 + iy + 1] - sbf[o2 + iy]) * wy;
+        acc += (val * u_value) * v_value;
     }
+    return acc;
+}
+
+// The envelope's real and imaginary parts for one component (range from launch_pack_coeffs).
+__device__ __forceinline__ void shapelet_sum(const PackedCoeff *__restrict__ packed, int4 range,
+                                             const double *__restrict__ sbf, int ix, int iy, double wx, double wy,
+                                             double &er, double &ei) {
+    er = shapelet_part(packed + range.x, range.y, sbf, ix, iy, wx, wy);
+    ei = shapelet_part(packed + range.x + range.y, range.z, sbf, ix, iy, wx, wy);
 }
 
 __device__ __forceinline__ void load_jones(const JonesF64 *p, double j[8]) {

@@ -28,8 +28,8 @@ struct ShapeletChanArgs {
     const double4 *shape;
     const double *coh;
     const double2 *bl_uv;     // [ncomp][nbl]
-    const int2 *coeff_range;
-    const ShapeletCoeff *coeffs;
+    const int4 *prange;
+    const PackedCoeff *packed;
     const double *sbf;
     BeamDesc beam;
     JonesF32 *vis32;

@@ -139,8 +139,9 @@ typedef struct SmInfo {
  * V^e = (K (.) C^e) K^H with K_pc = exp(i 2 pi (u_p l + v_p m + w_p (n - 1)) f / c), phases and
  * their range reduction in FP64, operands split into hi + lo halves of `dtype`, products on
  * MFMA with FP32 accumulation promoted to FP64 every 1024 components. Gaussians and
- * shapelets take the FP64 direct kernels. Requires one beam shared by every tile (or no
- * beam); with per-tile beams the points fall back to the FP64 direct kernel. */
+ * shapelets take the FP64 direct kernels. With per-tile beams, the baselines whose two tiles
+ * share the most common beam take the MFMA path and the other baselines the FP64 direct
+ * kernel. */
 enum { SM_MFMA_FP16 = 0, SM_MFMA_BF16 = 1 };
 const char *sm_model_timestep_factored(SmModeller *m, double lst, double latitude, int32_t dtype,
                                        JonesF32 *d_vis_fb, JonesF64 *d_vis_fb_f64);
@@ -148,6 +149,12 @@ const char *sm_model_timestep_factored(SmModeller *m, double lst, double latitud
 /* FP32 accumulation span of the split-precision kernel, in steps of 16 components (default
  * 64 = 1024 components): after each span the partial sums are added into FP64. Process-wide. */
 const char *sm_set_factored_segment(int32_t steps);
+
+/* Work-group organisation of the split-precision kernel. 1 (default): wave-specialised, four
+ * waves issue only MFMAs while eight generate the next step's operands; 0: four waves that
+ * each do both. Both compute the same operands and products in the same order, so their
+ * results agree bit for bit. Process-wide. */
+const char *sm_set_factored_kernel(int32_t kernel);
 
 /* Self-test of the MFMA lane layout the factored kernel assumes: D (32 x 32 f32, row-major)
  * = A (32 x 16 f16, row-major) . B (16 x 32 f16, row-major), one wave, device pointers. */

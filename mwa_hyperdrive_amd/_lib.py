@@ -146,6 +146,7 @@ SIGNATURES = {
     "sm_model_timestep_factored": [_P, _D, _D, _I, _P, _P],
     "sm_selftest_mfma_layout": [_P, _P, _P],
     "sm_set_factored_segment": [C.c_int32],
+    "sm_set_factored_kernel": [C.c_int32],
     "sm_surrogate_new": [C.c_uint64, C.POINTER(_P)],
     "sm_surrogate_get_params": [_P, _P],
     "sm_surrogate_set_params": [_P, _P],

@@ -62,6 +62,10 @@ template <> struct Mfma<__bf16> {
 };
 
 template <class E> __device__ __forceinline__ void split(float x, E &hi, E &lo) {
+    // x is first rounded to FP32: the empty asm keeps the compiler from folding the
+    // producing FMA and this conversion into one mixed-precision FMA that rounds once to
+    // 16 bits, which would make the split depend on instruction selection.
+    asm("" : "+v"(x));
     hi = (E)x;
     lo = (E)(x - (float)hi);
 }
@@ -117,6 +121,15 @@ __global__ __launch_bounds__(256) void k_factored_coh32(const FactoredArgs a) {
     reinterpret_cast<float *>(a.cohf)[i] = v;
 }
 
+// FP32 complex product K C with the rounding fixed by explicit fused multiply-adds, so that
+// every kernel organisation produces the same operands bit for bit.
+__device__ __forceinline__ float cmul_re(float kr, float ki, float cr, float ci) {
+    return __builtin_fmaf(kr, cr, -(ki * ci));
+}
+__device__ __forceinline__ float cmul_im(float kr, float ki, float cr, float ci) {
+    return __builtin_fmaf(kr, ci, ki * cr);
+}
+
 // A thread's operands for one step: K_p and K_q of its 4 components (FP32 complex pairs)
 // and their scaled coherencies for the work-group's two Jones entries.
 struct Ops {
@@ -146,8 +159,8 @@ __device__ __forceinline__ void generate(const Ops &g, StepLds<E> &L, int part, 
             const float cr = e ? g.c[j].z : g.c[j].x;
             const float ci = e ? g.c[j].w : g.c[j].y;
             E h0, l0, h1, l1;
-            split<E>(kr[j] * cr - ki[j] * ci, h0, l0);
-            split<E>(kr[j] * ci + ki[j] * cr, h1, l1);
+            split<E>(cmul_re(kr[j], ki[j], cr, ci), h0, l0);
+            split<E>(cmul_im(kr[j], ki[j], cr, ci), h1, l1);
             h[2 * j] = h0;
             h[2 * j + 1] = h1;
             l[2 * j] = l0;
@@ -327,6 +340,190 @@ __global__ __launch_bounds__(256, 2) void k_factored(const FactoredArgs a) {
         factored_block<E, false>(a, buf, f, blk, half, pq);
 }
 
+// ---------------------------------------------------------------- wave-specialised variant
+// Four consumer waves (one per SIMD) run only the MFMAs of step s from one LDS buffer while
+// kProd producer waves generate step s + 1 into the other; one barrier per step. Each SIMD
+// thus always holds a wave with MFMAs to issue and one with ALU work, instead of relying on
+// two general waves to drift out of phase. Register use is the larger of the two roles.
+constexpr int kProdWaves = 8;
+
+// A producer thread's operands for one step: 2 components.
+struct Ops2 {
+    float4 p, q, c[2];
+};
+
+template <class E>
+__device__ __forceinline__ void generate2(const Ops2 &g, StepLds<E> &L, int gr, int g2) {
+    typedef E v4 __attribute__((ext_vector_type(4)));
+    const float kr[2] = {g.p.x, g.p.z};
+    const float ki[2] = {g.p.y, g.p.w};
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        v4 h, l;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const float cr = e ? g.c[j].z : g.c[j].x;
+            const float ci = e ? g.c[j].w : g.c[j].y;
+            E h0, l0, h1, l1;
+            split<E>(cmul_re(kr[j], ki[j], cr, ci), h0, l0);
+            split<E>(cmul_im(kr[j], ki[j], cr, ci), h1, l1);
+            h[2 * j] = h0;
+            h[2 * j + 1] = h1;
+            l[2 * j] = l0;
+            l[2 * j + 1] = l1;
+        }
+        *reinterpret_cast<v4 *>(&L.a[0][e][lds_at(gr, 2 * g2)]) = h;
+        *reinterpret_cast<v4 *>(&L.a[1][e][lds_at(gr, 2 * g2)]) = l;
+    }
+    const float qr[2] = {g.q.x, g.q.z};
+    const float qi[2] = {g.q.y, g.q.w};
+    v4 h0, l0, h1, l1;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        E rh, rl, ih, il;
+        split<E>(qr[j], rh, rl);
+        split<E>(qi[j], ih, il);
+        h0[2 * j] = rh;
+        h0[2 * j + 1] = ih;
+        l0[2 * j] = rl;
+        l0[2 * j + 1] = il;
+        h1[2 * j] = -ih;
+        h1[2 * j + 1] = rh;
+        l1[2 * j] = -il;
+        l1[2 * j + 1] = rl;
+    }
+    *reinterpret_cast<v4 *>(&L.b[0][lds_at(gr, 2 * g2)]) = h0;
+    *reinterpret_cast<v4 *>(&L.b[1][lds_at(gr, 2 * g2)]) = l0;
+    *reinterpret_cast<v4 *>(&L.b[0][lds_at(kFB + gr, 2 * g2)]) = h1;
+    *reinterpret_cast<v4 *>(&L.b[1][lds_at(kFB + gr, 2 * g2)]) = l1;
+}
+
+template <class E, bool DIAG>
+__device__ __forceinline__ void factored_ws_block(const FactoredArgs &a, StepLds<E> (&buf)[2], int f, int blk,
+                                                  int half, int2 pq) {
+    typedef typename Mfma<E>::vec V;
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6;
+    const int nsteps = a.cpad / kFC;
+    constexpr bool kBfLoLo = std::is_same<E, __bf16>::value;
+    if (wave >= 4) {
+        // Producers: thread t handles tile t / 8 of each block side and components
+        // 2 (t % 8), 2 (t % 8) + 1 of a step; two register sets alternate so every load has
+        // two steps to arrive.
+        static_assert(kProdWaves * 64 == 512, "producer mapping assumes 512 threads");
+        const int pt = tid - 256;
+        const int gr = pt >> 3;
+        const int g2 = (pt & 7) * 2;
+        const float2 *kf = a.kbuf + (size_t)f * a.tpad * a.cpad;
+        const unsigned offp = (unsigned)((pq.x * kFB + gr) * a.cpad + g2);
+        const unsigned offq = (unsigned)((pq.y * kFB + gr) * a.cpad + g2);
+        const float4 *cf = a.cohf + (size_t)f * a.cpad * 2 + g2 * 2 + half;
+        auto load = [&](Ops2 &g, int step) {
+            const unsigned o = (unsigned)step * kFC;
+            g.p = *reinterpret_cast<const float4 *>(kf + offp + o);
+            g.q = *reinterpret_cast<const float4 *>(kf + offq + o);
+            g.c[0] = cf[2 * o];
+            g.c[1] = cf[2 * (o + 1)];
+        };
+        auto clamp_step = [&](int s) { return s < nsteps ? s : nsteps - 1; };
+        Ops2 g0, g1;
+        load(g0, 0);
+        load(g1, clamp_step(1));
+        generate2<E>(g0, buf[0], gr, g2);
+        load(g0, clamp_step(2));
+        for (int step = 0; step < nsteps; step += 2) {
+            // Step s + 1 from g1 while the consumers run step s; then step s + 2 from g0.
+            __syncthreads();
+            generate2<E>(g1, buf[1], gr, g2);
+            load(g1, clamp_step(step + 3));
+            if (step + 1 < nsteps) {
+                __syncthreads();
+                generate2<E>(g0, buf[0], gr, g2);
+                load(g0, clamp_step(step + 4));
+            }
+        }
+        return;
+    }
+    // Consumers: wave w computes local Jones entry w % 2 for output columns (w / 2) * 64 ..
+    const int lane = tid & 63;
+    const int le = wave & 1;
+    const int colbase = (wave >> 1) * kFB;
+    const int row = lane & 31;
+    const int kofs = (lane >> 5) * 8;
+    f16v acc[2][2], cor[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = cor[i][j][r] = 0.0f;
+    for (int step = 0; step < nsteps; ++step) {
+        const int cur = step & 1;
+        __syncthreads();
+#pragma unroll
+        for (int kb = 0; kb < kFK / 16; ++kb) {
+            V ahi[2], alo[2], bhi[2], blo[2];
+#pragma unroll
+            for (int t2 = 0; t2 < 2; ++t2) {
+                const int ia = lds_at(t2 * 32 + row, kb * 16 + kofs);
+                const int ib = lds_at(colbase + t2 * 32 + row, kb * 16 + kofs);
+                ahi[t2] = *reinterpret_cast<const V *>(&buf[cur].a[0][le][ia]);
+                alo[t2] = *reinterpret_cast<const V *>(&buf[cur].a[1][le][ia]);
+                bhi[t2] = *reinterpret_cast<const V *>(&buf[cur].b[0][ib]);
+                blo[t2] = *reinterpret_cast<const V *>(&buf[cur].b[1][ib]);
+            }
+#pragma unroll
+            for (int round = 0; round < (kBfLoLo ? 4 : 3); ++round)
+#pragma unroll
+                for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+                    for (int ct = 0; ct < 2; ++ct) {
+                        if (DIAG && rt == 1 && ct == 0) continue;
+                        if (round == 0) acc[rt][ct] = Mfma<E>::run(ahi[rt], bhi[ct], acc[rt][ct]);
+                        if (round == 1) cor[rt][ct] = Mfma<E>::run(ahi[rt], blo[ct], cor[rt][ct]);
+                        if (round == 2) cor[rt][ct] = Mfma<E>::run(alo[rt], bhi[ct], cor[rt][ct]);
+                        if (round == 3) cor[rt][ct] = Mfma<E>::run(alo[rt], blo[ct], cor[rt][ct]);
+                    }
+        }
+        if ((step + 1) % a.seg_steps == 0 || step + 1 == nsteps) {
+            double *out = a.out + ((size_t)(f * a.nblocks + blk) * 4 + 2 * half) * (kFB * 2 * kFB);
+            unsigned lo = (unsigned)(le * (kFB * 2 * kFB) + (4 * (lane >> 5)) * (2 * kFB) + colbase + (lane & 31));
+            asm volatile("" : "+v"(lo));
+#pragma unroll
+            for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+                for (int ct = 0; ct < 2; ++ct) {
+                    if (DIAG && rt == 1 && ct == 0) continue;
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        const int o = (rt * 32 + 8 * (r >> 2) + (r & 3)) * (2 * kFB) + ct * 32;
+                        unsafeAtomicAdd(&out[lo + o], (double)acc[rt][ct][r] + (double)cor[rt][ct][r]);
+                        acc[rt][ct][r] = 0.0f;
+                        cor[rt][ct][r] = 0.0f;
+                        if ((r & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+                    }
+                }
+        }
+    }
+}
+
+template <class E>
+__global__ __launch_bounds__(256 + kProdWaves * 64) void k_factored_ws(const FactoredArgs a) {
+    __shared__ __attribute__((aligned(16))) StepLds<E> buf[2];
+    const int total = a.nchan * a.nblocks * 2;
+    int b = blockIdx.x;
+    if ((total & 7) == 0) b = (b & 7) * (total >> 3) + (b >> 3);
+    const int f = b / (2 * a.nblocks);
+    const int rem = b - f * 2 * a.nblocks;
+    const int blk = rem >> 1;
+    const int half = rem & 1;
+    const int2 pq = a.blocks[blk];
+    if (pq.x == pq.y)
+        factored_ws_block<E, true>(a, buf, f, blk, half, pq);
+    else
+        factored_ws_block<E, false>(a, buf, f, blk, half, pq);
+}
+
 // FP64 block results -> visibilities [freq][baseline] (+=), undoing the scale. map[i] =
 // (baseline, p, q): the output baseline and its two tiles in the factored tile numbering.
 __global__ void k_factored_scatter(const FactoredArgs a, const int4 *__restrict__ map, int nmap, int nbl,
@@ -410,6 +607,8 @@ std::vector<int2> factored_blocks(int ntiles, std::vector<int> &block_of) {
 
 size_t factored_out_doubles(int nchan, int nblocks) { return (size_t)nchan * nblocks * 4 * kFB * 2 * kFB; }
 
+int g_factored_kernel = 1;
+
 int factored_chunk(int nchan, int ntiles, int ncomp, int seg_steps) {
     const size_t tpad = (size_t)((ntiles + kFB - 1) / kFB) * kFB;
     const size_t per_comp = (size_t)nchan * tpad * sizeof(float2);
@@ -450,10 +649,17 @@ hipError_t launch_factored(int dtype, FactoredArgs a, int chunk, const double *c
         hipLaunchKernelGGL(k_factored_phasors, dim3((unsigned)((nph + 255) / 256)), dim3(256), 0, stream, a);
         const size_t ncf = (size_t)a.nchan * a.cpad * 8;
         hipLaunchKernelGGL(k_factored_coh32, dim3((unsigned)((ncf + 255) / 256)), dim3(256), 0, stream, a);
-        if (dtype == SM_MFMA_BF16)
+        if (g_factored_kernel == 1) {
+            const dim3 bs(256 + kProdWaves * 64);
+            if (dtype == SM_MFMA_BF16)
+                hipLaunchKernelGGL(k_factored_ws<__bf16>, dim3(total), bs, 0, stream, a);
+            else
+                hipLaunchKernelGGL(k_factored_ws<_Float16>, dim3(total), bs, 0, stream, a);
+        } else if (dtype == SM_MFMA_BF16) {
             hipLaunchKernelGGL(k_factored<__bf16>, dim3(total), dim3(256), 0, stream, a);
-        else
+        } else {
             hipLaunchKernelGGL(k_factored<_Float16>, dim3(total), dim3(256), 0, stream, a);
+        }
     }
     if (ev1 != nullptr) (void)hipEventRecord(ev1, stream);
     hipError_t e = hipGetLastError();

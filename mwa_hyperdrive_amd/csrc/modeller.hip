@@ -627,6 +627,12 @@ extern "C" const char *sm_set_factored_segment(int32_t steps) {
     return nullptr;
 }
 
+extern "C" const char *sm_set_factored_kernel(int32_t kernel) {
+    if (kernel != 0 && kernel != 1) return "factored kernel must be 0 or 1";
+    g_factored_kernel = kernel;
+    return nullptr;
+}
+
 // One timestep on the device. factored_dtype < 0: FP64 direct kernels for every type;
 // otherwise points go through the split-precision MFMA path in that operand type.
 static const char *model_timestep_device(SmModeller *m, double lst, double latitude, const UVW *d_uvws,

@@ -24,6 +24,9 @@ struct FactoredArgs {
     double *out;              // [nchan][nblocks][4][64][128] FP64 block results
 };
 
+// 1 (default): wave-specialised MFMA kernel; 0: every wave generates operands and runs MFMAs.
+extern int g_factored_kernel;
+
 std::vector<int2> factored_blocks(int ntiles, std::vector<int> &block_of);
 size_t factored_out_doubles(int nchan, int nblocks);
 // Components per chunk (a multiple of 16 * seg_steps unless one chunk holds them all), so

@@ -2,8 +2,8 @@
 
 Bar: relative visibility error <= 1e-6 of max|V| against the FP64 oracle for fp16 hi+lo
 operands. bf16 hi+lo keeps 16 mantissa bits; its error is measured and reported, with a
-looser bound written here (5e-6). Per-tile beams fall back to the FP64 direct kernel and
-must then meet the FP64 bar."""
+looser bound written here (5e-6). With per-tile beams, baselines outside the most common
+beam row take the FP64 direct kernel and must then meet the FP64 bar."""
 
 import ctypes as C
 
@@ -111,3 +111,25 @@ def test_factored_c3_full_timestep_vs_fp64_direct():
         err = diff.norm(dim=1).max().item() / scale
         print(f"C3 full timestep {name}: rel err vs FP64 direct {err:.3e}")
         assert err <= tol
+
+
+@pytest.mark.parametrize("n_comps", [10, 23, 40])
+def test_factored_kernel_organisations_agree(n_comps):
+    """The wave-specialised kernel and the all-waves-generate kernel compute the same operands
+    and products in the same order: FP64 outputs equal bit for bit, for an odd and an even
+    number of 16-component steps, diagonal and off-diagonal tile blocks (70 tiles), and FP32
+    spans of 1, 3 and 64 steps."""
+    obs = synthetic.shrink(synthetic.config(3, scale=n_comps / 20000), tiles_n=70, chans=5, timesteps=1)
+    try:
+        for dtype in (0, 1):
+            for seg in (1, 3, 64):
+                _lib.call("sm_set_factored_segment", seg)
+                outs = []
+                for kernel in (0, 1):
+                    _lib.call("sm_set_factored_kernel", kernel)
+                    outs.append(factored(obs, 0, dtype)[1])
+                assert np.array_equal(outs[0], outs[1]), (dtype, seg)
+                assert np.isfinite(outs[0]).all()
+    finally:
+        _lib.call("sm_set_factored_segment", 64)
+        _lib.call("sm_set_factored_kernel", 1)

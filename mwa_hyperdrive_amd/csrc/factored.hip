@@ -21,6 +21,10 @@
 //     product) is below the other errors and is not computed; bf16 adds it. Every `seg_steps`
 //     steps the FP32 partials are added into an FP64 buffer. Two work-groups share a CU, so
 //     one group's operand generation overlaps the other's MFMAs.
+//   * k_factored_ws (the default) does the same work with the roles split between waves:
+//     per work-group, 8 producer waves generate the operands of step s + 1 into LDS while 4
+//     consumer waves (one per SIMD) run the MFMAs of step s. The products, their order and
+//     the results are those of k_factored, bit for bit.
 // A scatter kernel maps the FP64 block results to baselines and accumulates visibilities.
 #include <algorithm>
 #include <cmath>
@@ -43,7 +47,7 @@ constexpr int kFK = 2 * kFC;        // real K per step
 // chunk c of row r lives at chunk c ^ swz(r). A 16-byte access by 16 lanes then touches 16
 // distinct 4-bank groups both for the fragment reads (16 rows, one chunk) and for the
 // generation writes (4 rows x 4 chunks).
-__device__ __forceinline__ int swz(int row) { return (row >> 2) & 3; }
+__device__ __forceinline__ int swz(int row) { return kFK == 32 ? (row >> 2) & 3 : row & 7; }
 __device__ __forceinline__ int lds_at(int row, int k) { return row * kFK + ((((k >> 3) ^ swz(row)) << 3) | (k & 7)); }
 constexpr size_t kKbufCap = size_t(4) << 30;  // phasor buffer bytes per chunk
 
@@ -61,14 +65,37 @@ template <> struct Mfma<__bf16> {
     }
 };
 
-template <class E> __device__ __forceinline__ void split(float x, E &hi, E &lo) {
-    // x is first rounded to FP32: the empty asm keeps the compiler from folding the
-    // producing FMA and this conversion into one mixed-precision FMA that rounds once to
-    // 16 bits, which would make the split depend on instruction selection.
-    asm("" : "+v"(x));
-    hi = (E)x;
-    lo = (E)(x - (float)hi);
+// Splits (x0, x1) into hi = rn_E(x) and lo = rn_E(x - hi), each pair packed in one 32-bit
+// word (x0 in the low half). The empty asm rounds x to FP32 first: it keeps the compiler from
+// folding the producing FMA and the conversion into one mixed-precision FMA that rounds once
+// to 16 bits, which would make the split depend on instruction selection.
+template <class E> __device__ __forceinline__ void split_words(float x0, float x1, unsigned &hi, unsigned &lo);
+
+// fp16: one packed conversion for hi, then lo = rn16(x - hi) as one mixed-precision FMA per
+// element (x * 1 - hi, hi read as fp16 from its half of the packed word). x - hi is exact in
+// FP32, so the FMA's single rounding gives the same lo as rounding twice.
+template <> __device__ __forceinline__ void split_words<_Float16>(float x0, float x1, unsigned &hi, unsigned &lo) {
+    asm("" : "+v"(x0));
+    asm("" : "+v"(x1));
+    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+    hi = __builtin_bit_cast(unsigned, h2{(_Float16)x0, (_Float16)x1});
+    asm("v_fma_mixlo_f16 %0, %1, 1.0, -%2 op_sel_hi:[0,0,1]" : "=v"(lo) : "v"(x0), "v"(hi));
+    asm("v_fma_mixhi_f16 %0, %1, 1.0, -%2 op_sel:[0,0,1] op_sel_hi:[0,0,1]" : "+v"(lo) : "v"(x1), "v"(hi));
 }
+
+// bf16: packed round-to-nearest-even conversions; hi widens back to FP32 by a shift (low
+// half) and a mask (high half).
+template <> __device__ __forceinline__ void split_words<__bf16>(float x0, float x1, unsigned &hi, unsigned &lo) {
+    asm("" : "+v"(x0));
+    asm("" : "+v"(x1));
+    asm("v_cvt_pk_bf16_f32 %0, %1, %2" : "=v"(hi) : "v"(x0), "v"(x1));
+    const float d0 = x0 - __builtin_bit_cast(float, hi << 16);
+    const float d1 = x1 - __builtin_bit_cast(float, hi & 0xffff0000u);
+    asm("v_cvt_pk_bf16_f32 %0, %1, %2" : "=v"(lo) : "v"(d0), "v"(d1));
+}
+
+// (re, im) -> (-im, re) on a packed pair of 16-bit floats.
+__device__ __forceinline__ unsigned swap_neg(unsigned w) { return ((w >> 16) | (w << 16)) ^ 0x8000u; }
 
 // K[f][t][c] for the chunk's components; tiles >= ntiles and components past the chunk
 // hold 0, so the MFMA kernel needs no bounds checks.
@@ -148,51 +175,38 @@ template <class E> struct StepLds {
 // MFMAs. Thread t handles tile t / 4 of each block side and components 4 (t % 4) .. + 3.
 template <class E>
 __device__ __forceinline__ void generate(const Ops &g, StepLds<E> &L, int part, int gr, int gq) {
-    typedef typename Mfma<E>::vec V;
     const float kr[4] = {g.p[0].x, g.p[0].z, g.p[1].x, g.p[1].z};
     const float ki[4] = {g.p[0].y, g.p[0].w, g.p[1].y, g.p[1].w};
     {
         const int e = part;
-        V h, l;
+        uint4 h, l;
+        unsigned *hw = &h.x, *lw = &l.x;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const float cr = e ? g.c[j].z : g.c[j].x;
             const float ci = e ? g.c[j].w : g.c[j].y;
-            E h0, l0, h1, l1;
-            split<E>(cmul_re(kr[j], ki[j], cr, ci), h0, l0);
-            split<E>(cmul_im(kr[j], ki[j], cr, ci), h1, l1);
-            h[2 * j] = h0;
-            h[2 * j + 1] = h1;
-            l[2 * j] = l0;
-            l[2 * j + 1] = l1;
+            split_words<E>(cmul_re(kr[j], ki[j], cr, ci), cmul_im(kr[j], ki[j], cr, ci), hw[j], lw[j]);
         }
-        *reinterpret_cast<V *>(&L.a[0][e][lds_at(gr, 2 * gq)]) = h;
-        *reinterpret_cast<V *>(&L.a[1][e][lds_at(gr, 2 * gq)]) = l;
+        *reinterpret_cast<uint4 *>(&L.a[0][e][lds_at(gr, 2 * gq)]) = h;
+        *reinterpret_cast<uint4 *>(&L.a[1][e][lds_at(gr, 2 * gq)]) = l;
     }
     if (part == 0) return;
     // Column gr carries K_q: rows (2c, 2c+1) hold (K_re, K_im) for V_re and (-K_im, K_re)
     // for V_im.
     const float qr[4] = {g.q[0].x, g.q[0].z, g.q[1].x, g.q[1].z};
     const float qi[4] = {g.q[0].y, g.q[0].w, g.q[1].y, g.q[1].w};
-    V h0, l0, h1, l1;
+    uint4 h0, l0, h1, l1;
+    unsigned *h0w = &h0.x, *l0w = &l0.x, *h1w = &h1.x, *l1w = &l1.x;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        E rh, rl, ih, il;
-        split<E>(qr[j], rh, rl);
-        split<E>(qi[j], ih, il);
-        h0[2 * j] = rh;
-        h0[2 * j + 1] = ih;
-        l0[2 * j] = rl;
-        l0[2 * j + 1] = il;
-        h1[2 * j] = -ih;
-        h1[2 * j + 1] = rh;
-        l1[2 * j] = -il;
-        l1[2 * j + 1] = rl;
+        split_words<E>(qr[j], qi[j], h0w[j], l0w[j]);
+        h1w[j] = swap_neg(h0w[j]);
+        l1w[j] = swap_neg(l0w[j]);
     }
-    *reinterpret_cast<V *>(&L.b[0][lds_at(gr, 2 * gq)]) = h0;
-    *reinterpret_cast<V *>(&L.b[1][lds_at(gr, 2 * gq)]) = l0;
-    *reinterpret_cast<V *>(&L.b[0][lds_at(kFB + gr, 2 * gq)]) = h1;
-    *reinterpret_cast<V *>(&L.b[1][lds_at(kFB + gr, 2 * gq)]) = l1;
+    *reinterpret_cast<uint4 *>(&L.b[0][lds_at(gr, 2 * gq)]) = h0;
+    *reinterpret_cast<uint4 *>(&L.b[1][lds_at(gr, 2 * gq)]) = l0;
+    *reinterpret_cast<uint4 *>(&L.b[0][lds_at(kFB + gr, 2 * gq)]) = h1;
+    *reinterpret_cast<uint4 *>(&L.b[1][lds_at(kFB + gr, 2 * gq)]) = l1;
 }
 
 // Four waves: wave w computes local Jones entry w % 2 (entry 2 * half + w % 2) for output
@@ -342,10 +356,9 @@ __global__ __launch_bounds__(256, 2) void k_factored(const FactoredArgs a) {
 
 // ---------------------------------------------------------------- wave-specialised variant
 // Four consumer waves (one per SIMD) run only the MFMAs of step s from one LDS buffer while
-// kProd producer waves generate step s + 1 into the other; one barrier per step. Each SIMD
+// 4 or 8 producer waves generate step s + 1 into the other; one barrier per step. Each SIMD
 // thus always holds a wave with MFMAs to issue and one with ALU work, instead of relying on
 // two general waves to drift out of phase. Register use is the larger of the two roles.
-constexpr int kProdWaves = 8;
 
 // A producer thread's operands for one step: 2 components.
 struct Ops2 {
@@ -354,97 +367,124 @@ struct Ops2 {
 
 template <class E>
 __device__ __forceinline__ void generate2(const Ops2 &g, StepLds<E> &L, int gr, int g2) {
-    typedef E v4 __attribute__((ext_vector_type(4)));
     const float kr[2] = {g.p.x, g.p.z};
     const float ki[2] = {g.p.y, g.p.w};
 #pragma unroll
     for (int e = 0; e < 2; ++e) {
-        v4 h, l;
+        uint2 h, l;
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
             const float cr = e ? g.c[j].z : g.c[j].x;
             const float ci = e ? g.c[j].w : g.c[j].y;
-            E h0, l0, h1, l1;
-            split<E>(cmul_re(kr[j], ki[j], cr, ci), h0, l0);
-            split<E>(cmul_im(kr[j], ki[j], cr, ci), h1, l1);
-            h[2 * j] = h0;
-            h[2 * j + 1] = h1;
-            l[2 * j] = l0;
-            l[2 * j + 1] = l1;
+            split_words<E>(cmul_re(kr[j], ki[j], cr, ci), cmul_im(kr[j], ki[j], cr, ci), j ? h.y : h.x,
+                           j ? l.y : l.x);
         }
-        *reinterpret_cast<v4 *>(&L.a[0][e][lds_at(gr, 2 * g2)]) = h;
-        *reinterpret_cast<v4 *>(&L.a[1][e][lds_at(gr, 2 * g2)]) = l;
+        *reinterpret_cast<uint2 *>(&L.a[0][e][lds_at(gr, 2 * g2)]) = h;
+        *reinterpret_cast<uint2 *>(&L.a[1][e][lds_at(gr, 2 * g2)]) = l;
     }
-    const float qr[2] = {g.q.x, g.q.z};
-    const float qi[2] = {g.q.y, g.q.w};
-    v4 h0, l0, h1, l1;
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-        E rh, rl, ih, il;
-        split<E>(qr[j], rh, rl);
-        split<E>(qi[j], ih, il);
-        h0[2 * j] = rh;
-        h0[2 * j + 1] = ih;
-        l0[2 * j] = rl;
-        l0[2 * j + 1] = il;
-        h1[2 * j] = -ih;
-        h1[2 * j + 1] = rh;
-        l1[2 * j] = -il;
-        l1[2 * j + 1] = rl;
-    }
-    *reinterpret_cast<v4 *>(&L.b[0][lds_at(gr, 2 * g2)]) = h0;
-    *reinterpret_cast<v4 *>(&L.b[1][lds_at(gr, 2 * g2)]) = l0;
-    *reinterpret_cast<v4 *>(&L.b[0][lds_at(kFB + gr, 2 * g2)]) = h1;
-    *reinterpret_cast<v4 *>(&L.b[1][lds_at(kFB + gr, 2 * g2)]) = l1;
+    uint2 h0, l0;
+    split_words<E>(g.q.x, g.q.y, h0.x, l0.x);
+    split_words<E>(g.q.z, g.q.w, h0.y, l0.y);
+    const uint2 h1{swap_neg(h0.x), swap_neg(h0.y)};
+    const uint2 l1{swap_neg(l0.x), swap_neg(l0.y)};
+    *reinterpret_cast<uint2 *>(&L.b[0][lds_at(gr, 2 * g2)]) = h0;
+    *reinterpret_cast<uint2 *>(&L.b[1][lds_at(gr, 2 * g2)]) = l0;
+    *reinterpret_cast<uint2 *>(&L.b[0][lds_at(kFB + gr, 2 * g2)]) = h1;
+    *reinterpret_cast<uint2 *>(&L.b[1][lds_at(kFB + gr, 2 * g2)]) = l1;
 }
 
+// Fragments of one 16-deep K slice for a consumer wave: A rows 0..31 and 32..63 of its Jones
+// entry and B columns colbase + 0..31 and + 32..63, hi and lo halves.
+template <class E> struct Frags {
+    typename Mfma<E>::vec ahi[2], alo[2], bhi[2], blo[2];
+};
+
+template <class E>
+__device__ __forceinline__ void read_frags(Frags<E> &F, const StepLds<E> &L, int kb, int le, int colbase, int row,
+                                           int kofs) {
+    typedef typename Mfma<E>::vec V;
+#pragma unroll
+    for (int t2 = 0; t2 < 2; ++t2) {
+        const int ia = lds_at(t2 * 32 + row, kb * 16 + kofs);
+        const int ib = lds_at(colbase + t2 * 32 + row, kb * 16 + kofs);
+        F.ahi[t2] = *reinterpret_cast<const V *>(&L.a[0][le][ia]);
+        F.alo[t2] = *reinterpret_cast<const V *>(&L.a[1][le][ia]);
+        F.bhi[t2] = *reinterpret_cast<const V *>(&L.b[0][ib]);
+        F.blo[t2] = *reinterpret_cast<const V *>(&L.b[1][ib]);
+    }
+}
+
+// Product rounds over the 2 x 2 tiles (hi*hi into acc; hi*lo, lo*hi and, for bf16, lo*lo
+// into cor), so that MFMAs writing the same accumulator are never back to back.
 template <class E, bool DIAG>
+__device__ __forceinline__ void mfma_rounds(const Frags<E> &F, f16v (&acc)[2][2], f16v (&cor)[2][2]) {
+    constexpr bool kBfLoLo = std::is_same<E, __bf16>::value;
+#pragma unroll
+    for (int round = 0; round < (kBfLoLo ? 4 : 3); ++round)
+#pragma unroll
+        for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct) {
+                if (DIAG && rt == 1 && ct == 0) continue;
+                if (round == 0) acc[rt][ct] = Mfma<E>::run(F.ahi[rt], F.bhi[ct], acc[rt][ct]);
+                if (round == 1) cor[rt][ct] = Mfma<E>::run(F.ahi[rt], F.blo[ct], cor[rt][ct]);
+                if (round == 2) cor[rt][ct] = Mfma<E>::run(F.alo[rt], F.bhi[ct], cor[rt][ct]);
+                if (round == 3) cor[rt][ct] = Mfma<E>::run(F.alo[rt], F.blo[ct], cor[rt][ct]);
+            }
+}
+
+// Wave-specialised block: 4 consumer waves run the MFMAs of step s from one LDS buffer while 8
+// producer waves generate step s + 1 into the other. Producer thread t handles tile t / 8 of
+// each block side and components 2 (t % 8), 2 (t % 8) + 1 of a step. Its phasor and coherency
+// loads are issued D steps ahead, in D register sets used in turn, since each step's loads
+// miss in the caches.
+template <class E, bool DIAG, int D>
 __device__ __forceinline__ void factored_ws_block(const FactoredArgs &a, StepLds<E> (&buf)[2], int f, int blk,
                                                   int half, int2 pq) {
-    typedef typename Mfma<E>::vec V;
+    static_assert(D >= 2 && D % 2 == 0, "an even number of register sets");
+    static_assert(kFC == 16 && kFB == 64, "512 producer threads cover 64 rows x 16 components");
     const int tid = threadIdx.x;
     const int wave = tid >> 6;
     const int nsteps = a.cpad / kFC;
-    constexpr bool kBfLoLo = std::is_same<E, __bf16>::value;
     if (wave >= 4) {
-        // Producers: thread t handles tile t / 8 of each block side and components
-        // 2 (t % 8), 2 (t % 8) + 1 of a step; two register sets alternate so every load has
-        // two steps to arrive.
-        static_assert(kProdWaves * 64 == 512, "producer mapping assumes 512 threads");
         const int pt = tid - 256;
         const int gr = pt >> 3;
-        const int g2 = (pt & 7) * 2;
+        const int gc = (pt & 7) * 2;
         const float2 *kf = a.kbuf + (size_t)f * a.tpad * a.cpad;
-        const unsigned offp = (unsigned)((pq.x * kFB + gr) * a.cpad + g2);
-        const unsigned offq = (unsigned)((pq.y * kFB + gr) * a.cpad + g2);
-        const float4 *cf = a.cohf + (size_t)f * a.cpad * 2 + g2 * 2 + half;
+        const unsigned offp = (unsigned)((pq.x * kFB + gr) * a.cpad + gc);
+        const unsigned offq = (unsigned)((pq.y * kFB + gr) * a.cpad + gc);
+        const float4 *cf = a.cohf + (size_t)f * a.cpad * 2 + gc * 2 + half;
+        // Loads past the last step are clamped to it; they fill buffers no consumer reads.
         auto load = [&](Ops2 &g, int step) {
+            step = step < nsteps ? step : nsteps - 1;
             const unsigned o = (unsigned)step * kFC;
             g.p = *reinterpret_cast<const float4 *>(kf + offp + o);
             g.q = *reinterpret_cast<const float4 *>(kf + offq + o);
             g.c[0] = cf[2 * o];
             g.c[1] = cf[2 * (o + 1)];
         };
-        auto clamp_step = [&](int s) { return s < nsteps ? s : nsteps - 1; };
-        Ops2 g0, g1;
-        load(g0, 0);
-        load(g1, clamp_step(1));
-        generate2<E>(g0, buf[0], gr, g2);
-        load(g0, clamp_step(2));
-        for (int step = 0; step < nsteps; step += 2) {
-            // Step s + 1 from g1 while the consumers run step s; then step s + 2 from g0.
-            __syncthreads();
-            generate2<E>(g1, buf[1], gr, g2);
-            load(g1, clamp_step(step + 3));
-            if (step + 1 < nsteps) {
+        // Step 0 goes in before the first barrier; after barrier s the producers write step
+        // s + 1 into buffer (s + 1) % 2 from register set (s + 1) % D, then reload that set
+        // with step s + 1 + D.
+        Ops2 g[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) load(g[j], j);
+        generate2<E>(g[0], buf[0], gr, gc);
+        load(g[0], D);
+        for (int s = 0; s < nsteps; s += D) {
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                if (s + d >= nsteps) break;
                 __syncthreads();
-                generate2<E>(g0, buf[0], gr, g2);
-                load(g0, clamp_step(step + 4));
+                const int j = (d + 1) % D;
+                generate2<E>(g[j], buf[(d + 1) & 1], gr, gc);
+                load(g[j], s + d + 1 + D);
             }
         }
         return;
     }
     // Consumers: wave w computes local Jones entry w % 2 for output columns (w / 2) * 64 ..
+    // + 63 (V_re columns, then V_im columns), 2 x 2 tiles of 32 x 32.
     const int lane = tid & 63;
     const int le = wave & 1;
     const int colbase = (wave >> 1) * kFB;
@@ -457,35 +497,18 @@ __device__ __forceinline__ void factored_ws_block(const FactoredArgs &a, StepLds
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = cor[i][j][r] = 0.0f;
+    Frags<E> F;
     for (int step = 0; step < nsteps; ++step) {
-        const int cur = step & 1;
         __syncthreads();
 #pragma unroll
         for (int kb = 0; kb < kFK / 16; ++kb) {
-            V ahi[2], alo[2], bhi[2], blo[2];
-#pragma unroll
-            for (int t2 = 0; t2 < 2; ++t2) {
-                const int ia = lds_at(t2 * 32 + row, kb * 16 + kofs);
-                const int ib = lds_at(colbase + t2 * 32 + row, kb * 16 + kofs);
-                ahi[t2] = *reinterpret_cast<const V *>(&buf[cur].a[0][le][ia]);
-                alo[t2] = *reinterpret_cast<const V *>(&buf[cur].a[1][le][ia]);
-                bhi[t2] = *reinterpret_cast<const V *>(&buf[cur].b[0][ib]);
-                blo[t2] = *reinterpret_cast<const V *>(&buf[cur].b[1][ib]);
-            }
-#pragma unroll
-            for (int round = 0; round < (kBfLoLo ? 4 : 3); ++round)
-#pragma unroll
-                for (int rt = 0; rt < 2; ++rt)
-#pragma unroll
-                    for (int ct = 0; ct < 2; ++ct) {
-                        if (DIAG && rt == 1 && ct == 0) continue;
-                        if (round == 0) acc[rt][ct] = Mfma<E>::run(ahi[rt], bhi[ct], acc[rt][ct]);
-                        if (round == 1) cor[rt][ct] = Mfma<E>::run(ahi[rt], blo[ct], cor[rt][ct]);
-                        if (round == 2) cor[rt][ct] = Mfma<E>::run(alo[rt], bhi[ct], cor[rt][ct]);
-                        if (round == 3) cor[rt][ct] = Mfma<E>::run(alo[rt], blo[ct], cor[rt][ct]);
-                    }
+            read_frags<E>(F, buf[step & 1], kb, le, colbase, row, kofs);
+            mfma_rounds<E, DIAG>(F, acc, cor);
         }
         if ((step + 1) % a.seg_steps == 0 || step + 1 == nsteps) {
+            // Output layout of v_mfma_f32_32x32x16: element r of lane l is row
+            // 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32; no-return FP64 atomics, the
+            // work-group being the only writer of its region.
             double *out = a.out + ((size_t)(f * a.nblocks + blk) * 4 + 2 * half) * (kFB * 2 * kFB);
             unsigned lo = (unsigned)(le * (kFB * 2 * kFB) + (4 * (lane >> 5)) * (2 * kFB) + colbase + (lane & 31));
             asm volatile("" : "+v"(lo));
@@ -507,8 +530,8 @@ __device__ __forceinline__ void factored_ws_block(const FactoredArgs &a, StepLds
     }
 }
 
-template <class E>
-__global__ __launch_bounds__(256 + kProdWaves * 64) void k_factored_ws(const FactoredArgs a) {
+template <class E, int D>
+__global__ __launch_bounds__(768) void k_factored_ws(const FactoredArgs a) {
     __shared__ __attribute__((aligned(16))) StepLds<E> buf[2];
     const int total = a.nchan * a.nblocks * 2;
     int b = blockIdx.x;
@@ -519,9 +542,9 @@ __global__ __launch_bounds__(256 + kProdWaves * 64) void k_factored_ws(const Fac
     const int half = rem & 1;
     const int2 pq = a.blocks[blk];
     if (pq.x == pq.y)
-        factored_ws_block<E, true>(a, buf, f, blk, half, pq);
+        factored_ws_block<E, true, D>(a, buf, f, blk, half, pq);
     else
-        factored_ws_block<E, false>(a, buf, f, blk, half, pq);
+        factored_ws_block<E, false, D>(a, buf, f, blk, half, pq);
 }
 
 // FP64 block results -> visibilities [freq][baseline] (+=), undoing the scale. map[i] =
@@ -650,11 +673,10 @@ hipError_t launch_factored(int dtype, FactoredArgs a, int chunk, const double *c
         const size_t ncf = (size_t)a.nchan * a.cpad * 8;
         hipLaunchKernelGGL(k_factored_coh32, dim3((unsigned)((ncf + 255) / 256)), dim3(256), 0, stream, a);
         if (g_factored_kernel == 1) {
-            const dim3 bs(256 + kProdWaves * 64);
             if (dtype == SM_MFMA_BF16)
-                hipLaunchKernelGGL(k_factored_ws<__bf16>, dim3(total), bs, 0, stream, a);
+                hipLaunchKernelGGL((k_factored_ws<__bf16, 4>), dim3(total), dim3(768), 0, stream, a);
             else
-                hipLaunchKernelGGL(k_factored_ws<_Float16>, dim3(total), bs, 0, stream, a);
+                hipLaunchKernelGGL((k_factored_ws<_Float16, 4>), dim3(total), dim3(768), 0, stream, a);
         } else if (dtype == SM_MFMA_BF16) {
             hipLaunchKernelGGL(k_factored<__bf16>, dim3(total), dim3(256), 0, stream, a);
         } else {

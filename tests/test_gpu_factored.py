@@ -115,10 +115,10 @@ def test_factored_c3_full_timestep_vs_fp64_direct():
 
 @pytest.mark.parametrize("n_comps", [10, 23, 40])
 def test_factored_kernel_organisations_agree(n_comps):
-    """The wave-specialised kernel and the all-waves-generate kernel compute the same operands
-    and products in the same order: FP64 outputs equal bit for bit, for an odd and an even
-    number of 16-component steps, diagonal and off-diagonal tile blocks (70 tiles), and FP32
-    spans of 1, 3 and 64 steps."""
+    """Both work-group organisations of the split-precision kernel (every wave generates and
+    multiplies; 8 producer waves feed 4 MFMA waves) compute the same operands and products in
+    the same order: FP64 outputs equal bit for bit, for 1, 2 and 3 steps of 16 components,
+    diagonal and off-diagonal tile blocks (70 tiles), and FP32 spans of 1, 3 and 64 steps."""
     obs = synthetic.shrink(synthetic.config(3, scale=n_comps / 20000), tiles_n=70, chans=5, timesteps=1)
     try:
         for dtype in (0, 1):

@@ -274,7 +274,7 @@ __global__ __launch_bounds__(256) void k_shapelet_chan(const ShapeletChanArgs a)
                 for (int k = 0; k < cnt; ++k) {
                     double val;
                     int n1, n2;
-                    load_packed(cp + k, val, n1, n2);
+                    load_packed_uniform(cp + k, val, n1, n2);
                     const double xl = wx_lds[wv][n1][jx], xh = wx_lds[wv][n1][jx + 1];
                     const double yl = wy_lds[wv][n2][jy], yh = wy_lds[wv][n2][jy + 1];
                     const double u_value = xl + (xh - xl) * wx;

@@ -7,6 +7,16 @@ namespace sm {
 
 // One packed coefficient as a single 16-byte load (a scalar load when the address is
 // wave-uniform).
+// The same for a wave-uniform p, through the constant address space so that the compiler
+// issues scalar loads (the coefficients are never written while a kernel runs).
+__device__ __forceinline__ void load_packed_uniform(const PackedCoeff *p, double &value, int &n1, int &n2) {
+    typedef __attribute__((address_space(4))) const int cint;
+    const cint *q = (const cint *)p;
+    value = __hiloint2double(q[1], q[0]);
+    n1 = q[2];
+    n2 = q[3];
+}
+
 __device__ __forceinline__ void load_packed(const PackedCoeff *p, double &value, int &n1, int &n2) {
     const int4 w = *reinterpret_cast<const int4 *>(p);
     value = __hiloint2double(w.y, w.x);

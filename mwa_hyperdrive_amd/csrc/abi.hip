@@ -122,6 +122,13 @@ hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf
     a.nsub = ctx.nsub;
     if (ev0 != nullptr) (void)hipEventRecord(ev0, stream);
     if (ctx.d_bl_subset != nullptr) {
+        a.csplit = direct_subset_split(ctx.nsub, ctx.nchan, n);
+        if (a.csplit > 1) {
+            thread_local DevBuf accum;
+            e = accum.reserve(sizeof(double) * 8 * (size_t)ctx.nchan * ctx.nsub);
+            if (e != hipSuccess) return e;
+            a.accum = accum.as<double>();
+        }
         e = launch_direct(kind, per_tile, ctx.even, a, stream);
     } else if (g_direct_variant == 0 && kind != KIND_SHAPELET && !per_tile && ctx.even && ctx.d_tile_uvw != nullptr &&
         ctx.d_pairs != nullptr && ctx.ntiles >= 2) {

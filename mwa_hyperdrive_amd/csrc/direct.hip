@@ -13,6 +13,8 @@
 // the Gaussian envelope exp(q f_k^2) advances by a geometric recurrence in the same way.
 // Blocks are remapped so that all baseline blocks of a channel chunk land on the same XCD
 // and share that XCD's L2 copy of the chunk's coherency rows.
+#include <algorithm>
+
 #include "sm_internal.h"
 #include "sm_jones.h"
 
@@ -143,10 +145,20 @@ __device__ __forceinline__ void logical_block(int gx, int total, int &bx, int &c
     bx = b - chunk * gx;
 }
 
-template <int KIND, bool PER_TILE, bool EVEN>
+template <int KIND, bool PER_TILE, bool EVEN, bool SPLIT>
 __global__ __launch_bounds__(256) void k_direct(const DirectArgs a, int gx, int total) {
     int bx, chunk;
     logical_block(gx, total, bx, chunk);
+    // Component range: all, or one of csplit ranges (subset mode).
+    int c_lo = 0, c_hi = a.ncomp;
+    if (SPLIT) {
+        const int gy = (a.nchan + SM_NCH - 1) / SM_NCH;
+        const int split = chunk / gy;
+        chunk -= split * gy;
+        const int per = (a.ncomp + a.csplit - 1) / a.csplit;
+        c_lo = split * per;
+        c_hi = min(a.ncomp, c_lo + per);
+    }
     // Lane -> baseline: all baselines, or the listed subset (outputs keep the full layout).
     const int nlanes = a.bl_subset != nullptr ? a.nsub : a.nbl;
     const int bl_raw = bx * 256 + (int)threadIdx.x;
@@ -174,7 +186,7 @@ __global__ __launch_bounds__(256) void k_direct(const DirectArgs a, int gx, int 
 #pragma unroll
         for (int e = 0; e < 8; ++e) acc[k][e] = 0.0;
 
-    for (int c = 0; c < a.ncomp; ++c) {
+    for (int c = c_lo; c < c_hi; ++c) {
         const double4 lmn = lmn4[c];
         const double theta = uvw.u * lmn.x + uvw.v * lmn.y + uvw.w * lmn.z;
         double zr = 1.0, zi = 0.0, rr = 1.0, ri = 0.0;
@@ -255,6 +267,17 @@ __global__ __launch_bounds__(256) void k_direct(const DirectArgs a, int gx, int 
         }
     }
     if (bl_raw >= nlanes) return;
+    if (SPLIT) {
+#pragma unroll
+        for (int k = 0; k < SM_NCH; ++k) {
+            const int f = f0 + k;
+            if (f >= a.nchan) break;
+            double *dst = a.accum + ((size_t)f * a.nsub + li) * 8;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) unsafeAtomicAdd(&dst[e], acc[k][e]);
+        }
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < SM_NCH; ++k) {
         const int f = f0 + k;
@@ -275,11 +298,16 @@ template <int KIND, bool PT>
 static hipError_t launch_direct_kind(const DirectArgs &a, bool even, hipStream_t stream) {
     const int gx = ((a.bl_subset != nullptr ? a.nsub : a.nbl) + 255) / 256;
     const int gy = (a.nchan + SM_NCH - 1) / SM_NCH;
-    const int total = gx * gy;
-    if (even)
-        hipLaunchKernelGGL((k_direct<KIND, PT, true>), dim3(total), dim3(256), 0, stream, a, gx, total);
+    const int total = gx * gy * a.csplit;
+    // The split form is a separate instantiation so that the common kernels compile as before.
+    if (a.csplit > 1 && even)
+        hipLaunchKernelGGL((k_direct<KIND, PT, true, true>), dim3(total), dim3(256), 0, stream, a, gx, total);
+    else if (a.csplit > 1)
+        hipLaunchKernelGGL((k_direct<KIND, PT, false, true>), dim3(total), dim3(256), 0, stream, a, gx, total);
+    else if (even)
+        hipLaunchKernelGGL((k_direct<KIND, PT, true, false>), dim3(total), dim3(256), 0, stream, a, gx, total);
     else
-        hipLaunchKernelGGL((k_direct<KIND, PT, false>), dim3(total), dim3(256), 0, stream, a, gx, total);
+        hipLaunchKernelGGL((k_direct<KIND, PT, false, false>), dim3(total), dim3(256), 0, stream, a, gx, total);
     return hipGetLastError();
 }
 
@@ -294,9 +322,58 @@ bool chunks_even(const double *freqs, int nchan) {
     return true;
 }
 
-hipError_t launch_direct(Kind kind, bool per_tile, bool even, const DirectArgs &a, hipStream_t stream) {
+static hipError_t launch_direct_switch(Kind kind, bool per_tile, bool even, const DirectArgs &a, hipStream_t stream);
+
+__global__ void k_zero_f64(size_t n, double *x) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        x[i] = 0.0;
+}
+
+// vis[f][subset[i]] += accum[f][i], FP64 and its f32 demotion.
+__global__ void k_subset_finish(int nchan, int nsub, int nbl, const int *__restrict__ subset,
+                                const double *__restrict__ accum, JonesF32 *vis32, JonesF64 *vis64) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)nchan * nsub) return;
+    const int f = (int)(i / nsub);
+    const int s = (int)(i % nsub);
+    const size_t o = (size_t)f * nbl + subset[s];
+    const double *src = accum + i * 8;
+    float *v32 = reinterpret_cast<float *>(&vis32[o]);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v32[e] += (float)src[e];
+    if (vis64 != nullptr) {
+        double *v64 = reinterpret_cast<double *>(&vis64[o]);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v64[e] += src[e];
+    }
+}
+
+int direct_subset_split(int nsub, int nchan, int ncomp) {
+    const long lanes = (long)nsub * ((nchan + SM_NCH - 1) / SM_NCH);
+    const long want = 1L << 18;  // about four waves per SIMD on 256 CUs
+    long split = (want + lanes - 1) / std::max(lanes, 1L);
+    split = std::min<long>(split, std::max(1, ncomp / 32));
+    return (int)std::max(1L, std::min(split, 64L));
+}
+
+hipError_t launch_direct(Kind kind, bool per_tile, bool even, const DirectArgs &a_in, hipStream_t stream) {
+    DirectArgs a = a_in;
     if (a.ncomp <= 0 || a.nbl <= 0 || a.nchan <= 0) return hipSuccess;
     if (a.bl_subset != nullptr && a.nsub <= 0) return hipSuccess;
+    if (a.bl_subset == nullptr || a.accum == nullptr) a.csplit = 1;
+    if (a.csplit > 1) {
+        const size_t n = (size_t)a.nchan * a.nsub * 8;
+        hipLaunchKernelGGL(k_zero_f64, dim3(1024), dim3(256), 0, stream, n, a.accum);
+    }
+    hipError_t e = launch_direct_switch(kind, per_tile, even, a, stream);
+    if (e != hipSuccess || a.csplit <= 1) return e;
+    const size_t n = (size_t)a.nchan * a.nsub;
+    hipLaunchKernelGGL(k_subset_finish, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, a.nchan, a.nsub,
+                       a.nbl, a.bl_subset, a.accum, a.vis32, a.vis64);
+    return hipGetLastError();
+}
+
+static hipError_t launch_direct_switch(Kind kind, bool per_tile, bool even, const DirectArgs &a, hipStream_t stream) {
     switch (kind) {
     case KIND_POINT:
         return per_tile ? launch_direct_kind<KIND_POINT, true>(a, even, stream)

@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "sm_factored.h"
+#include "sm_jones.h"
 #include "sm_math.h"
 
 namespace sm {
@@ -148,6 +149,54 @@ __global__ __launch_bounds__(256) void k_factored_coh32(const FactoredArgs a) {
     reinterpret_cast<float *>(a.cohf)[i] = v;
 }
 
+// Per-tile beams: cohf[f][r][c][8] = J_r S J_0^H * scale for class r (beam row class_row[r])
+// and every component of the chunk; coh holds S.
+__device__ __forceinline__ void class_coherency(const FactoredArgs &a, int r, int c, int f, double o[8]) {
+    double sv[8], ja[8], jb[8];
+    load_jones(reinterpret_cast<const JonesF64 *>(a.coh + ((size_t)c * a.nchan_pad + f) * 8), sv);
+    const BeamDesc &b = a.beam;
+    const size_t col = (size_t)b.freq_map[f] * b.n_dir + b.dir_offset + c;
+    load_jones(&b.jones[(size_t)a.class_row[r] * b.n_ufreq * b.n_dir + col], ja);
+    load_jones(&b.jones[(size_t)a.class_row[0] * b.n_ufreq * b.n_dir + col], jb);
+    sandwich(ja, sv, jb, o);
+}
+
+__global__ __launch_bounds__(256) void k_factored_coh32_classes(const FactoredArgs a) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (size_t)a.nchan * a.nclass * a.cpad) return;
+    const int cl = (int)(i % a.cpad);
+    const size_t fr = i / a.cpad;
+    const int r = (int)(fr % a.nclass);
+    const int f = (int)(fr / a.nclass);
+    float *dst = reinterpret_cast<float *>(a.cohf) + i * 8;
+    if (cl >= a.c_count) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) dst[k] = 0.0f;
+        return;
+    }
+    double o[8];
+    class_coherency(a, r, a.c_begin + cl, f, o);
+    const double sc = *a.scale;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) dst[k] = (float)(o[k] * sc);
+}
+
+// max |J_r S J_0^H| over classes, components and channels, for the scale.
+__global__ void k_absmax_classes(const FactoredArgs a, unsigned long long *out) {
+    double m = 0.0;
+    const size_t n = (size_t)a.nclass * a.ncomp * a.nchan;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int f = (int)(i % a.nchan);
+        const size_t rc = i / a.nchan;
+        double o[8];
+        class_coherency(a, (int)(rc / a.ncomp), (int)(rc % a.ncomp), f, o);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) m = fmax(m, fabs(o[k]));
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+}
+
 // FP32 complex product K C with the rounding fixed by explicit fused multiply-adds, so that
 // every kernel organisation produces the same operands bit for bit.
 __device__ __forceinline__ float cmul_re(float kr, float ki, float cr, float ci) {
@@ -233,7 +282,8 @@ __device__ __forceinline__ void factored_block(const FactoredArgs &a, StepLds<E>
     const unsigned offq = (unsigned)((pq.y * kFB + gr) * a.cpad + gq);
     // Scaled FP32 coherencies of the thread's 4 components: (re, im) of the work-group's first
     // Jones entry, then of its second, one float4 per component.
-    const float4 *cf = a.cohf + (size_t)f * a.cpad * 2 + gq * 2 + half;
+    const int rc = a.rclass != nullptr ? a.rclass[pq.x * kFB + gr] : 0;
+    const float4 *cf = a.cohf + ((size_t)f * a.nclass + rc) * a.cpad * 2 + gq * 2 + half;
     auto load = [&](Ops &g, int step) {
         const unsigned o = (unsigned)step * kFC;
         g.p[0] = *reinterpret_cast<const float4 *>(kf + offp + o);
@@ -453,7 +503,8 @@ __device__ __forceinline__ void factored_ws_block(const FactoredArgs &a, StepLds
         const float2 *kf = a.kbuf + (size_t)f * a.tpad * a.cpad;
         const unsigned offp = (unsigned)((pq.x * kFB + gr) * a.cpad + gc);
         const unsigned offq = (unsigned)((pq.y * kFB + gr) * a.cpad + gc);
-        const float4 *cf = a.cohf + (size_t)f * a.cpad * 2 + gc * 2 + half;
+        const int rc = a.rclass != nullptr ? a.rclass[pq.x * kFB + gr] : 0;
+        const float4 *cf = a.cohf + ((size_t)f * a.nclass + rc) * a.cpad * 2 + gc * 2 + half;
         // Loads past the last step are clamped to it; they fill buffers no consumer reads.
         auto load = [&](Ops2 &g, int step) {
             step = step < nsteps ? step : nsteps - 1;
@@ -559,12 +610,16 @@ __global__ void k_factored_scatter(const FactoredArgs a, const int4 *__restrict_
     const int blk = block_of[(e4.y / kFB) * gb + (e4.z / kFB)];
     const int pl = e4.y % kFB, ql = e4.z % kFB;
     const double inv = 1.0 / *a.scale;
+    // e4.w = 1: the baseline is the conjugate transpose of block result (p, q): entry (i, j)
+    // comes from entry (j, i), conjugated.
+    const bool herm = e4.w != 0;
     double v[8];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-        const double *o = a.out + ((size_t)(f * a.nblocks + blk) * 4 + e) * (kFB * 2 * kFB) + pl * (2 * kFB);
+        const int es = herm ? ((e & 1) << 1) | (e >> 1) : e;
+        const double *o = a.out + ((size_t)(f * a.nblocks + blk) * 4 + es) * (kFB * 2 * kFB) + pl * (2 * kFB);
         v[2 * e] = o[ql] * inv;
-        v[2 * e + 1] = o[kFB + ql] * inv;
+        v[2 * e + 1] = (herm ? -o[kFB + ql] : o[kFB + ql]) * inv;
     }
     const size_t vo = (size_t)f * nbl + e4.x;
     float *v32 = reinterpret_cast<float *>(&vis32[vo]);
@@ -577,14 +632,15 @@ __global__ void k_factored_scatter(const FactoredArgs a, const int4 *__restrict_
     }
 }
 
-// dst[i] = src[idx[i]] for rows of 3 doubles (tile UVWs of a tile subset).
+// dst[i] = src[idx[i]] for rows of 3 doubles (tile UVWs of a tile subset); idx[i] < 0 gives
+// zeros (padding tiles).
 __global__ void k_gather_rows3(int n, const int *__restrict__ idx, const double *__restrict__ src, double *dst) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int t = idx[i];
-    dst[3 * i] = src[3 * t];
-    dst[3 * i + 1] = src[3 * t + 1];
-    dst[3 * i + 2] = src[3 * t + 2];
+    dst[3 * i] = t >= 0 ? src[3 * t] : 0.0;
+    dst[3 * i + 1] = t >= 0 ? src[3 * t + 1] : 0.0;
+    dst[3 * i + 2] = t >= 0 ? src[3 * t + 2] : 0.0;
 }
 
 hipError_t launch_gather_rows3(int n, const int *d_idx, const double *d_src, double *d_dst, hipStream_t stream) {
@@ -647,7 +703,9 @@ size_t factored_kbuf_bytes(int nchan, int ntiles, int chunk) {
     return (size_t)nchan * tpad * chunk * sizeof(float2);
 }
 
-size_t factored_cohf_bytes(int nchan, int chunk) { return (size_t)nchan * chunk * 8 * sizeof(float); }
+size_t factored_cohf_bytes(int nchan, int chunk, int nclass) {
+    return (size_t)nchan * nclass * chunk * 8 * sizeof(float);
+}
 
 hipError_t launch_factored(int dtype, FactoredArgs a, int chunk, const double *coh_for_scale, size_t coh_n,
                            unsigned long long *d_max_bits, double *d_scale, const int4 *map, int nmap, int nbl,
@@ -657,7 +715,10 @@ hipError_t launch_factored(int dtype, FactoredArgs a, int chunk, const double *c
     if (chunk <= 0 || chunk % kFC != 0) return hipErrorInvalidValue;
     a.tpad = (a.ntiles + kFB - 1) / kFB * kFB;
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, d_max_bits);
-    hipLaunchKernelGGL(k_absmax, dim3(1024), dim3(256), 0, stream, coh_n, coh_for_scale, d_max_bits);
+    if (a.nclass > 1)
+        hipLaunchKernelGGL(k_absmax_classes, dim3(1024), dim3(256), 0, stream, a, d_max_bits);
+    else
+        hipLaunchKernelGGL(k_absmax, dim3(1024), dim3(256), 0, stream, coh_n, coh_for_scale, d_max_bits);
     hipLaunchKernelGGL(k_scale_from_max, dim3(1), dim3(1), 0, stream, d_max_bits, d_scale);
     a.scale = d_scale;
     const size_t nout = factored_out_doubles(a.nchan, a.nblocks);
@@ -671,7 +732,12 @@ hipError_t launch_factored(int dtype, FactoredArgs a, int chunk, const double *c
         const size_t nph = (size_t)a.tpad * a.cpad;
         hipLaunchKernelGGL(k_factored_phasors, dim3((unsigned)((nph + 255) / 256)), dim3(256), 0, stream, a);
         const size_t ncf = (size_t)a.nchan * a.cpad * 8;
-        hipLaunchKernelGGL(k_factored_coh32, dim3((unsigned)((ncf + 255) / 256)), dim3(256), 0, stream, a);
+        if (a.nclass > 1) {
+            const size_t ncl = (size_t)a.nchan * a.nclass * a.cpad;
+            hipLaunchKernelGGL(k_factored_coh32_classes, dim3((unsigned)((ncl + 255) / 256)), dim3(256), 0, stream, a);
+        } else {
+            hipLaunchKernelGGL(k_factored_coh32, dim3((unsigned)((ncf + 255) / 256)), dim3(256), 0, stream, a);
+        }
         if (g_factored_kernel == 1) {
             if (dtype == SM_MFMA_BF16)
                 hipLaunchKernelGGL((k_factored_ws<__bf16, 4>), dim3(total), dim3(768), 0, stream, a);

@@ -214,11 +214,16 @@ struct SmModeller {
     // split-precision point path
     DevBuf d_fblocks, d_fblock_of, d_fout, d_fscratch, d_fkbuf, d_fcohf, d_fmap;
     int nfblocks = 0;
-    // Per-tile beams: the tiles of the most common beam row (group 0) share one coherency
-    // J0 S J0^H, so their pairs take the MFMA path; every other baseline takes the FP64
-    // direct kernel through the baseline subset d_g0_rest.
-    int g0_row = -1, g0_n = 0, g0_nblocks = 0, g0_nmap = 0, g0_nrest = 0;
+    // Per-tile beams. Class 0 is the most common beam row; every other beam row is a class
+    // of its own. Factored tiles: class-0 tiles first, padded to a 64-tile block boundary,
+    // then the other tiles. A pair with at least one class-0 tile q takes the MFMA path
+    // with row coherency J_r S J_0^H (r = the class of the other tile p): blocks class-0 x
+    // class-0 and other x class-0, the latter giving V_pq, or its conjugate transpose when
+    // the baseline's first tile is the class-0 one. Pairs of two other-class tiles take the
+    // FP64 direct kernel through the baseline subset d_g0_rest.
+    int g0_row = -1, g0_n = 0, g0_nblocks = 0, g0_nmap = 0, g0_nrest = 0, g0_nclass = 1;
     DevBuf d_g0_tiles, d_g0_blocks, d_g0_block_of, d_g0_map, d_g0_rest, d_g0_uvw, d_g0_coh;
+    DevBuf d_g0_rclass, d_g0_class_row;
     int npairs = 0;
     std::vector<double> h_tile_uvw;
     DevType types[3];
@@ -550,40 +555,76 @@ extern "C" const char *sm_new(const SmObsParams *p, const SmComponent *comps, co
         for (int b = 0; b < m->nbl; ++b) fmap[b] = make_int4(b, bl_tiles[b].x, bl_tiles[b].y, 0);
         SM_NEW_TRY(upload(m->d_fmap, fmap));
         if (m->n_utile > 1) {
-            // Group 0 = the beam row shared by the most tiles.
+            // Class 0 = the beam row shared by the most tiles.
             std::vector<int> count(m->n_utile, 0);
             for (int t = 0; t < m->nt; ++t) count[m->tile_rows[t]]++;
             int r0 = 0;
             for (int r = 1; r < m->n_utile; ++r)
                 if (count[r] > count[r0]) r0 = r;
-            std::vector<int> g0_tiles, local(m->nt, -1);
-            for (int t = 0; t < m->nt; ++t)
-                if (m->tile_rows[t] == r0) {
-                    local[t] = (int)g0_tiles.size();
-                    g0_tiles.push_back(t);
-                }
-            if (g0_tiles.size() >= 2) {
-                std::vector<int4> g0_map;
+            if (count[r0] >= 2) {
+                std::vector<int> class_of_row(m->n_utile, -1), class_row;
+                class_of_row[r0] = 0;
+                class_row.push_back(r0);
+                for (int r = 0; r < m->n_utile; ++r)
+                    if (class_of_row[r] < 0 && count[r] > 0) {
+                        class_of_row[r] = (int)class_row.size();
+                        class_row.push_back(r);
+                    }
+                // Factored numbering: class-0 tiles, padding to a block boundary, the rest.
+                std::vector<int> ftile, local(m->nt, -1);
+                for (int t = 0; t < m->nt; ++t)
+                    if (m->tile_rows[t] == r0) {
+                        local[t] = (int)ftile.size();
+                        ftile.push_back(t);
+                    }
+                const int n0 = (int)ftile.size();
+                const int g0b = (n0 + 63) / 64;
+                ftile.resize((size_t)g0b * 64, -1);
+                for (int t = 0; t < m->nt; ++t)
+                    if (m->tile_rows[t] != r0) {
+                        local[t] = (int)ftile.size();
+                        ftile.push_back(t);
+                    }
+                const int nft = (int)ftile.size();
+                const int G = (nft + 63) / 64;
+                std::vector<int> rclass((size_t)G * 64, 0);
+                for (int t = 0; t < nft; ++t)
+                    if (ftile[t] >= 0) rclass[t] = class_of_row[m->tile_rows[ftile[t]]];
+                std::vector<int2> blocks;
+                std::vector<int> block_of((size_t)G * G, 0);
+                for (int p = 0; p < G; ++p)
+                    for (int q = 0; q < g0b; ++q) {
+                        if (p < g0b && q < p) continue;
+                        block_of[(size_t)p * G + q] = (int)blocks.size();
+                        blocks.push_back(make_int2(p, q));
+                    }
+                std::vector<int4> map;
                 std::vector<int> rest;
                 for (int b = 0; b < m->nbl; ++b) {
                     const int i = bl_tiles[b].x, j = bl_tiles[b].y;
-                    if (local[i] >= 0 && local[j] >= 0)
-                        g0_map.push_back(make_int4(b, local[i], local[j], 0));
+                    const bool ci = m->tile_rows[i] == r0, cj = m->tile_rows[j] == r0;
+                    if (ci && cj)
+                        map.push_back(make_int4(b, local[i], local[j], 0));
+                    else if (cj)
+                        map.push_back(make_int4(b, local[i], local[j], 0));
+                    else if (ci)
+                        map.push_back(make_int4(b, local[j], local[i], 1));
                     else
                         rest.push_back(b);
                 }
-                std::vector<int> g0_block_of;
-                const std::vector<int2> g0_blocks = factored_blocks((int)g0_tiles.size(), g0_block_of);
                 m->g0_row = r0;
-                m->g0_n = (int)g0_tiles.size();
-                m->g0_nblocks = (int)g0_blocks.size();
-                m->g0_nmap = (int)g0_map.size();
+                m->g0_n = nft;
+                m->g0_nblocks = (int)blocks.size();
+                m->g0_nmap = (int)map.size();
                 m->g0_nrest = (int)rest.size();
-                SM_NEW_TRY(upload(m->d_g0_tiles, g0_tiles));
-                SM_NEW_TRY(upload(m->d_g0_blocks, g0_blocks));
-                SM_NEW_TRY(upload(m->d_g0_block_of, g0_block_of));
-                SM_NEW_TRY(upload(m->d_g0_map, g0_map));
+                m->g0_nclass = (int)class_row.size();
+                SM_NEW_TRY(upload(m->d_g0_tiles, ftile));
+                SM_NEW_TRY(upload(m->d_g0_blocks, blocks));
+                SM_NEW_TRY(upload(m->d_g0_block_of, block_of));
+                SM_NEW_TRY(upload(m->d_g0_map, map));
                 SM_NEW_TRY(upload(m->d_g0_rest, rest));
+                SM_NEW_TRY(upload(m->d_g0_rclass, rclass));
+                SM_NEW_TRY(upload(m->d_g0_class_row, class_row));
             }
         }
     }
@@ -703,6 +744,7 @@ static const char *model_timestep_device(SmModeller *m, double lst, double latit
         const int2 *fblocks = m->d_fblocks.as<int2>();
         const int *fblock_of = m->d_fblock_of.as<int>();
         const int4 *fmap = m->d_fmap.as<int4>();
+        int nclass = 1;
         if (fact_uniform) {
             // Coherency J S J^H for the shared beam row.
             SM_TRY(d.coh.reserve(sizeof(double) * 8 * (size_t)n * m->nchan_pad));
@@ -718,13 +760,13 @@ static const char *model_timestep_device(SmModeller *m, double lst, double latit
                 if (const char *e = timed(ty, e0, e1)) return e;
                 SM_TRY(run_type(kinds[ty], in, rest, d.coh, d_vis_fb, d_vis_fb_f64, 0, e0, e1));
             }
-            // Group 0: coherency J0 S J0^H and its tiles' UVWs.
-            BeamDesc b0 = ctx.beam;
-            b0.uniform_row = m->g0_row;
+            // Pairs with a class-0 tile: S here, J_r S J_0^H formed per chunk by the factored
+            // path; UVWs of the factored tiles (zeros for padding).
             SM_TRY(m->d_g0_coh.reserve(sizeof(double) * 8 * (size_t)n * m->nchan_pad));
-            SM_TRY(launch_coherency(n, m->nchan, m->nchan_pad, ctx.d_freqs_pad, in.fb, b0, true,
+            SM_TRY(launch_coherency(n, m->nchan, m->nchan_pad, ctx.d_freqs_pad, in.fb, ctx.beam, false,
                                     m->d_g0_coh.as<double>(), 0));
             coh = m->d_g0_coh.as<double>();
+            nclass = m->g0_nclass;
             SM_TRY(m->d_g0_uvw.reserve(sizeof(double) * 3 * m->g0_n));
             SM_TRY(launch_gather_rows3(m->g0_n, m->d_g0_tiles.as<int>(), tile_uvw, m->d_g0_uvw.as<double>(), 0));
             fuvw = m->d_g0_uvw.as<double>();
@@ -750,6 +792,12 @@ static const char *model_timestep_device(SmModeller *m, double lst, double latit
         fa.lmn = in.lmn;
         fa.coh = coh;
         fa.scale = nullptr;
+        fa.nclass = nclass;
+        if (nclass > 1) {
+            fa.rclass = m->d_g0_rclass.as<int>();
+            fa.class_row = m->d_g0_class_row.as<int>();
+            fa.beam = ctx.beam;
+        }
         SM_TRY(m->d_fout.reserve(sizeof(double) * factored_out_doubles(m->nchan, fnblocks)));
         fa.out = m->d_fout.as<double>();
         SM_TRY(m->d_fscratch.reserve(64));
@@ -758,7 +806,7 @@ static const char *model_timestep_device(SmModeller *m, double lst, double latit
         const int chunk = factored_chunk(m->nchan, fnt, n, fa.seg_steps);
         SM_TRY(m->d_fkbuf.reserve(factored_kbuf_bytes(m->nchan, fnt, chunk)));
         fa.kbuf = m->d_fkbuf.as<float2>();
-        SM_TRY(m->d_fcohf.reserve(factored_cohf_bytes(m->nchan, chunk)));
+        SM_TRY(m->d_fcohf.reserve(factored_cohf_bytes(m->nchan, chunk, nclass)));
         fa.cohf = m->d_fcohf.as<float4>();
         SM_TRY(launch_factored(factored_dtype, fa, chunk, coh, (size_t)n * m->nchan_pad * 8, bits, scale, fmap, nmap,
                                m->nbl, fblock_of, d_vis_fb, d_vis_fb_f64, 0, e0, e1));

@@ -20,8 +20,15 @@ struct FactoredArgs {
     const double *coh;        // [ncomp][nchan_pad][8], J S J^H
     const double *scale;      // device scalar, set by launch_factored
     float2 *kbuf;             // [nchan][tpad][cpad] phasors of the chunk
-    float4 *cohf;             // [nchan][cpad][2] scaled FP32 coherencies of the chunk
+    float4 *cohf;             // [nchan][nclass][cpad][2] scaled FP32 coherencies of the chunk
     double *out;              // [nchan][nblocks][4][64][128] FP64 block results
+    // Per-tile beams: the A-side coherency of row tile p is J_r S J_0^H with r the beam row of
+    // p's class, J_0 that of class 0 (every B-side tile belongs to class 0), and `coh` holds
+    // S. With nclass == 1, `coh` holds the coherency itself and the fields below are unused.
+    int nclass = 1;
+    const int *rclass = nullptr;     // [tpad] class of each factored tile
+    const int *class_row = nullptr;  // [nclass] beam row of each class
+    BeamDesc beam{};
 };
 
 // 1 (default): wave-specialised MFMA kernel; 0: every wave generates operands and runs MFMAs.
@@ -33,10 +40,11 @@ size_t factored_out_doubles(int nchan, int nblocks);
 // that the phasor buffer stays within a fixed cap, and that buffer's size.
 int factored_chunk(int nchan, int ntiles, int ncomp, int seg_steps);
 size_t factored_kbuf_bytes(int nchan, int ntiles, int chunk);
-size_t factored_cohf_bytes(int nchan, int chunk);
+size_t factored_cohf_bytes(int nchan, int chunk, int nclass);
 hipError_t launch_gather_rows3(int n, const int *d_idx, const double *d_src, double *d_dst, hipStream_t stream);
-// map[nmap] = (output baseline, p, q) with p < q in the factored tile numbering (a.ntiles
-// tiles); nbl = the visibility buffer's baseline count.
+// map[nmap] = (output baseline, p, q, h) in the factored tile numbering (a.ntiles tiles):
+// the baseline takes block result (p, q), or with h = 1 its conjugate transpose; nbl = the
+// visibility buffer's baseline count. coh_n = the element count of `coh` (nclass == 1).
 hipError_t launch_factored(int dtype, FactoredArgs a, int chunk, const double *coh_for_scale, size_t coh_n,
                            unsigned long long *d_max_bits, double *d_scale, const int4 *map, int nmap, int nbl,
                            const int *block_of, JonesF32 *vis32, JonesF64 *vis64, hipStream_t stream,

@@ -73,11 +73,18 @@ struct DirectArgs {
     JonesF64 *vis64;          // nullable
     const int *bl_subset = nullptr;  // nullable: model only these baselines (nsub of them)
     int nsub = 0;
+    // Subset mode only: a few baselines give too few lanes to fill the GPU, so the component
+    // loop is split into csplit ranges whose partial sums meet in accum ([nchan][nsub][8],
+    // FP64, set up by launch_direct) before one pass adds them to the visibilities.
+    int csplit = 1;
+    double *accum = nullptr;
 };
 
 // Launchers (direct.hip). All enqueue on `stream` and return hipSuccess or an error.
 hipError_t launch_pack_coeffs(int ncomp, const int2 *d_range, const ShapeletCoeff *d_coeffs, int4 *d_prange,
                               PackedCoeff *d_packed, hipStream_t stream);
+// Component ranges for a baseline subset of nsub baselines (1 when they fill the GPU).
+int direct_subset_split(int nsub, int nchan, int ncomp);
 hipError_t launch_coherency(int ncomp, int nchan, int nchan_pad, const double *d_freqs, FluxBuckets fb,
                             BeamDesc beam, bool apply_beam, double *d_coh, hipStream_t stream);
 hipError_t launch_direct(Kind kind, bool per_tile, bool even, const DirectArgs &a, hipStream_t stream);

@@ -64,11 +64,14 @@ def test_factored_vs_oracle(beam, n_tiles):
         assert m.info()["num_points"] == obs.num_components
 
 
-@pytest.mark.parametrize("n_tiles,dead", [(20, [4]), (40, [3, 11, 25, 38]), (70, [0, 69])])
+@pytest.mark.parametrize("n_tiles,dead", [(20, [4]), (40, [3, 11, 25, 38]), (70, [0, 69]), (90, [1, 40, 64, 70, 89])])
 def test_factored_per_tile_beams(n_tiles, dead):
-    """Per-tile beams (tiles with one dead dipole each, as in the combined 256-antenna case):
-    pairs inside the most common beam row take the MFMA path, every baseline that touches
-    another row takes the FP64 direct kernel and must meet the FP64 bar on its own."""
+    """Per-tile beams (tiles with one dead dipole each, as in the combined 256-antenna case).
+    Every pair with a tile of the most common beam row takes the MFMA path: pairs inside that
+    row with coherency J0 S J0^H, pairs with one other-row tile r with J_r S J0^H (or its
+    conjugate transpose when the other-row tile comes second). Pairs of two other-row tiles
+    take the FP64 direct kernel and must meet the FP64 bar on their own. 90 tiles put other-row
+    tiles on both sides of the class-0 tiles and across a 64-tile block boundary."""
     obs = synthetic.shrink(synthetic.config(3, scale=100 / 20000), tiles_n=n_tiles, chans=9, timesteps=1)
     obs.beam = "analytic"
     obs.delays = np.zeros((n_tiles, 16), np.uint32)
@@ -78,18 +81,28 @@ def test_factored_per_tile_beams(n_tiles, dead):
     o32, o64 = oracle_model(obs, 0)
     v32, v64, _ = factored(obs, 0, 0)
     err = rel_err(v64, o64)
-    odd = np.zeros(obs.num_baselines, bool)
+    both = np.zeros(obs.num_baselines, bool)
+    one = np.zeros(obs.num_baselines, bool)
     b = 0
     for i in range(n_tiles):
         for j in range(i + 1, n_tiles):
-            odd[b] = i in dead or j in dead
+            both[b] = i in dead and j in dead
+            one[b] = (i in dead) != (j in dead)
             b += 1
     scale = np.sqrt((o64.reshape(-1, 2) ** 2).sum(1)).max()
-    d = v64[:, odd] - o64[:, odd]
-    err_rest = np.sqrt((d.reshape(-1, 2) ** 2).sum(1)).max() / scale
-    print(f"per-tile beams, {n_tiles} tiles, dead {dead}: rel err {err:.3e} (FP64 baselines {err_rest:.3e})")
+
+    def part(mask):
+        if not mask.any():
+            return 0.0
+        d = v64[:, mask] - o64[:, mask]
+        return np.sqrt((d.reshape(-1, 2) ** 2).sum(1)).max() / scale
+
+    err_one, err_both = part(one), part(both)
+    print(f"per-tile beams, {n_tiles} tiles, dead {dead}: rel err {err:.3e} "
+          f"(one other-row tile {err_one:.3e}, FP64 pairs {err_both:.3e})")
     assert err <= FP16_TOL
-    assert err_rest <= 1e-12
+    assert 0.0 < err_one <= FP16_TOL
+    assert err_both <= 1e-12
 
 
 def test_factored_c3_full_timestep_vs_fp64_direct():

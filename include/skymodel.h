@@ -139,9 +139,9 @@ typedef struct SmInfo {
  * V^e = (K (.) C^e) K^H with K_pc = exp(i 2 pi (u_p l + v_p m + w_p (n - 1)) f / c), phases and
  * their range reduction in FP64, operands split into hi + lo halves of `dtype`, products on
  * MFMA with FP32 accumulation promoted to FP64 every 1024 components. Gaussians and
- * shapelets take the FP64 direct kernels. With per-tile beams, the baselines whose two tiles
- * share the most common beam take the MFMA path and the other baselines the FP64 direct
- * kernel. */
+ * shapelets take the FP64 direct kernels. With per-tile beams, every baseline with a tile of
+ * the most common beam row takes the MFMA path (row coherency J_r S J_0^H), and pairs of two
+ * tiles of other rows take the FP64 direct kernel. */
 enum { SM_MFMA_FP16 = 0, SM_MFMA_BF16 = 1 };
 const char *sm_model_timestep_factored(SmModeller *m, double lst, double latitude, int32_t dtype,
                                        JonesF32 *d_vis_fb, JonesF64 *d_vis_fb_f64);

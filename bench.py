@@ -148,13 +148,70 @@ def split_precision() -> dict:
     base = res["fp64_direct"]["gcompvis_per_s_kernel"]
     for name in ("fp16_hi_lo", "bf16_hi_lo"):
         res[name]["speedup_vs_fp64_direct"] = res[name]["gcompvis_per_s_kernel"] / base
-    # MFMA rate of the fp16 path: executed dense work (3 tile blocks x 4 Jones entries x
-    # 64 x 128 outputs x 2 K per component x 2 FLOP x 4 hi/lo products per channel).
-    blocks = 3
-    flop = blocks * 4 * 64 * 128 * 2 * obs.num_components * 2 * 4 * len(obs.freqs) * len(obs.lsts)
+    # MFMA rate of the fp16 path, executed dense work: per channel and component, each 64 x 64
+    # tile block computes 4 Jones entries x 64 x 128 outputs = 32 output tiles of 32 x 32
+    # (diagonal blocks skip 8 of them), with 2 real K per component, 2 FLOP per MAC and 3
+    # hi/lo products (hi*hi, hi*lo, lo*hi).
+    g = (len(obs.xyzs) + 63) // 64
+    tiles = g * 24 + (g * (g - 1) // 2) * 32
+    flop = tiles * 32 * 32 * 2 * 2 * 3 * obs.num_components * len(obs.freqs) * len(obs.lsts)
     res["fp16_hi_lo"]["mfma_tflops_executed"] = flop / (res["fp16_hi_lo"]["kernel_ms"] * 1e-3) / 1e12
     res["fp16_hi_lo"]["mfma_peak_tflops"] = 2500.0
     del m, v32, ref
+    torch.cuda.empty_cache()
+    return res
+
+
+def combined_c5() -> dict:
+    """C5 (256 tiles, per-tile beams with dead dipoles, 50 000 points + 10 000 Gaussians + 500
+    shapelets, 384 channels): one timestep through the FP64 direct path and through the
+    split-precision path (fp16 points on MFMA by beam-row classes), after one warm-up call
+    each; error of the latter against the former."""
+    import ctypes as C
+
+    import torch
+
+    from mwa_hyperdrive_amd import _lib, synthetic
+    from mwa_hyperdrive_amd.modeller import AnalyticBeam, SkyModeller
+
+    obs = synthetic.config(5)
+    m = SkyModeller(AnalyticBeam(obs.delays, obs.gains), obs.srclist, 0, obs.xyzs, obs.freqs, [],
+                    obs.phase_centre, 0.0, obs.latitude)
+    shape = (len(obs.freqs), obs.num_baselines, 8)
+    v32 = torch.zeros(shape, dtype=torch.float32, device="cuda")
+    ref = torch.zeros(shape, dtype=torch.float64, device="cuda")
+    out = torch.zeros(shape, dtype=torch.float64, device="cuda")
+    lst = float(obs.lsts[3])
+    work = obs.num_components * obs.num_baselines * len(obs.freqs)
+
+    def fp64(d64):
+        m.model_timestep_with_device(lst, obs.latitude, v32, d64)
+
+    def mixed(d64):
+        _lib.call("sm_model_timestep_factored", m._h, lst, obs.latitude, 0, C.c_void_p(v32.data_ptr()),
+                  C.c_void_p(d64.data_ptr()))
+
+    res = {"workload": f"C5: {obs.num_baselines} baselines, {len(obs.freqs)} channels, "
+                       f"{obs.num_components} components, per-tile beams ({m.info()['num_unique_tiles']} beam rows), "
+                       "one timestep", "comp_vis": work}
+    for name, fn, buf in (("fp64_direct", fp64, ref), ("fp16_points_mfma", mixed, out)):
+        fn(buf)
+        torch.cuda.synchronize()
+        buf.zero_()
+        m.set_timing(True)
+        t0 = time.perf_counter()
+        fn(buf)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        info = m.info()
+        m.set_timing(False)
+        res[name] = {"wall_ms": wall * 1e3, "gcompvis_per_s_wall": work / wall / 1e9,
+                     "kernel_ms": {k: info[f"kernel_ms_{k}"] for k in ("points", "factored", "gaussians", "shapelets")}}
+        print(f"[bench] C5 {name}: {wall * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    scale = ref.reshape(-1, 2).norm(dim=1).max().item()
+    res["fp16_points_mfma"]["rel_err_vs_fp64"] = (out - ref).reshape(-1, 2).norm(dim=1).max().item() / scale
+    res["speedup_end_to_end"] = res["fp64_direct"]["wall_ms"] / res["fp16_points_mfma"]["wall_ms"]
+    del m, v32, ref, out
     torch.cuda.empty_cache()
     return res
 
@@ -352,6 +409,7 @@ def main() -> int:
     torch.cuda.empty_cache()
     if not args.no_split_precision:
         out["split_precision"] = split_precision()
+        out["combined_c5"] = combined_c5()
     if not args.no_surrogate:
         out["surrogate"] = surrogate_bench(obs)
     if not args.no_cpu_baseline:

@@ -88,6 +88,24 @@ hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf
     e = launch_coherency(n, ctx.nchan, ctx.nchan_pad, ctx.d_freqs_pad, in.fb, ctx.beam, !per_tile, coh.as<double>(),
                          stream);
     if (e != hipSuccess) return e;
+    // Per-tile shapelets with beam-row classes: J_r S J_0^H per class for the channel-per-lane
+    // kernel, when the table fits a cap. (Measured on the 256-tile case: that kernel runs 28%
+    // faster. The baseline-per-lane kernels, already at 256 VGPRs, ran 2.2x slower with the
+    // same lookup compiled in, so they keep forming J_p S J_q^H.)
+    BeamDesc beam = ctx.beam;
+    const size_t cls_bytes = (size_t)ctx.nclass * n * ctx.nchan_pad * 8 * sizeof(double);
+    if (kind == KIND_SHAPELET && g_direct_variant != 1 && ctx.d_bl_subset == nullptr && per_tile &&
+        ctx.nclass > 0 && ctx.d_class_row != nullptr && ctx.class_row0 >= 0 && cls_bytes <= (size_t(24) << 30)) {
+        thread_local DevBuf cls;
+        e = cls.reserve(cls_bytes);
+        if (e != hipSuccess) return e;
+        e = launch_class_coherency(ctx.nclass, n, ctx.nchan, ctx.nchan_pad, coh.as<double>(), ctx.beam, ctx.d_class_row,
+                                   cls.as<double>(), stream);
+        if (e != hipSuccess) return e;
+        beam.coh_cls = cls.as<double>();
+        beam.class_of_row = ctx.d_class_of_row;
+        beam.row0 = ctx.class_row0;
+    }
     // Shapelet coefficients in the kernels' packed form (per thread: handles are not shared).
     thread_local DevBuf prange, packed;
     const int4 *d_prange = nullptr;
@@ -115,7 +133,7 @@ hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf
     a.prange = d_prange;
     a.packed = d_packed;
     a.sbf = ctx.d_sbf;
-    a.beam = ctx.beam;
+    a.beam = beam;
     a.vis32 = vis32;
     a.vis64 = vis64;
     a.bl_subset = ctx.d_bl_subset;
@@ -165,7 +183,7 @@ hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf
         s.prange = d_prange;
         s.packed = d_packed;
         s.sbf = ctx.d_sbf;
-        s.beam = ctx.beam;
+        s.beam = beam;
         s.vis32 = vis32;
         s.vis64 = vis64;
         e = launch_shapelet_chan(per_tile, s, stream);

@@ -209,9 +209,19 @@ __global__ __launch_bounds__(256) void k_shapelet_chan(const ShapeletChanArgs a)
     const UVW uvw = a.uvw[bl];
     int2 rows = make_int2(0, 0);
     int uf = 0;
+    int cls = -1;
+    bool herm = false;
     if (PER_TILE) {
         rows = a.beam.bl_rows[bl];
         uf = a.beam.freq_map[f];
+        if (a.beam.coh_cls != nullptr) {
+            if (rows.y == a.beam.row0) {
+                cls = a.beam.class_of_row[rows.x];
+            } else if (rows.x == a.beam.row0) {
+                cls = a.beam.class_of_row[rows.y];
+                herm = true;
+            }
+        }
     }
     const PackedCoeff *__restrict__ packed = a.packed;
     const double *__restrict__ sbf = a.sbf;
@@ -296,7 +306,9 @@ __global__ __launch_bounds__(256) void k_shapelet_chan(const ShapeletChanArgs a)
         const double pi = zr * ei + zi * er;
         double cj[8];
         const double *s = a.coh + ((size_t)c * a.nchan_pad + f) * 8;
-        if (PER_TILE) {
+        if (PER_TILE && cls >= 0) {
+            class_coh(a.beam.coh_cls + (((size_t)cls * a.ncomp + c) * a.nchan_pad + f) * 8, herm, cj);
+        } else if (PER_TILE) {
             double sv[8], j1[8], j2[8];
 #pragma unroll
             for (int e = 0; e < 8; ++e) sv[e] = s[e];

@@ -223,7 +223,7 @@ struct SmModeller {
     // FP64 direct kernel through the baseline subset d_g0_rest.
     int g0_row = -1, g0_n = 0, g0_nblocks = 0, g0_nmap = 0, g0_nrest = 0, g0_nclass = 1;
     DevBuf d_g0_tiles, d_g0_blocks, d_g0_block_of, d_g0_map, d_g0_rest, d_g0_uvw, d_g0_coh;
-    DevBuf d_g0_rclass, d_g0_class_row;
+    DevBuf d_g0_rclass, d_g0_class_row, d_class_of_row;
     int npairs = 0;
     std::vector<double> h_tile_uvw;
     DevType types[3];
@@ -625,6 +625,7 @@ extern "C" const char *sm_new(const SmObsParams *p, const SmComponent *comps, co
                 SM_NEW_TRY(upload(m->d_g0_rest, rest));
                 SM_NEW_TRY(upload(m->d_g0_rclass, rclass));
                 SM_NEW_TRY(upload(m->d_g0_class_row, class_row));
+                SM_NEW_TRY(upload(m->d_class_of_row, class_of_row));
             }
         }
     }
@@ -705,6 +706,12 @@ static const char *model_timestep_device(SmModeller *m, double lst, double latit
     ctx.d_tile_uvw = tile_uvw;
     ctx.d_pairs = m->d_pairs.as<int4>();
     ctx.npairs = m->npairs;
+    if (m->g0_row >= 0) {
+        ctx.nclass = m->g0_nclass;
+        ctx.class_row0 = m->g0_row;
+        ctx.d_class_row = m->d_g0_class_row.as<int>();
+        ctx.d_class_of_row = m->d_class_of_row.as<int>();
+    }
     const Kind kinds[3] = {KIND_POINT, KIND_GAUSSIAN, KIND_SHAPELET};
     for (int ty = 0; ty < 3; ++ty) {
         DevType &d = m->types[ty];
@@ -756,6 +763,7 @@ static const char *model_timestep_device(SmModeller *m, double lst, double latit
             ModelCtx rest = ctx;
             rest.d_bl_subset = m->d_g0_rest.as<int>();
             rest.nsub = m->g0_nrest;
+            rest.nclass = 0;  // no baseline of the subset has a class-0 tile
             if (m->g0_nrest > 0) {
                 if (const char *e = timed(ty, e0, e1)) return e;
                 SM_TRY(run_type(kinds[ty], in, rest, d.coh, d_vis_fb, d_vis_fb_f64, 0, e0, e1));

@@ -30,6 +30,11 @@ struct ModelCtx {
     int npairs = 0;
     const int *d_bl_subset = nullptr;  // nullable: the direct kernels model only these baselines
     int nsub = 0;
+    // Per-tile beams, optional: beam-row classes (class 0 = row class_row[0]); the per-tile
+    // kernels then read J_r S J_0^H from a class table for baselines with a class-0 tile.
+    int nclass = 0, class_row0 = -1;     // class_row0 = the beam row of class 0
+    const int *d_class_row = nullptr;   // [nclass] beam row of each class
+    const int *d_class_of_row = nullptr;  // [n_utile]
 };
 
 // 0: pick the fastest kernel for the case; 1: always the baseline-per-lane kernels.

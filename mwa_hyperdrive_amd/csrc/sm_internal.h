@@ -46,6 +46,13 @@ struct BeamDesc {
     int uniform_row;        // unique-tile row used by every tile (uniform mode), else -1
     const int2 *bl_rows;    // [nbl] unique-tile rows of (tile1, tile2) (per-tile mode)
     const int *tile_rows;   // [ntiles] unique-tile row per unflagged tile (autos)
+    // Per-tile mode, optional: J_r S J_0^H for every beam-row class r, where class 0 is row
+    // row0 ([nclass][ncomp][nchan_pad][8]). In the channel-per-lane shapelet kernel, a
+    // baseline with one tile in row row0 reads its coherency here (conjugate-transposed when
+    // row0 is its first tile's row) instead of forming J_p S J_q^H per visibility.
+    const double *coh_cls = nullptr;
+    const int *class_of_row = nullptr;  // [n_utile]
+    int row0 = -1;
 };
 
 // A shapelet coefficient as the kernels read it: the value with the sign of
@@ -83,6 +90,10 @@ struct DirectArgs {
 // Launchers (direct.hip). All enqueue on `stream` and return hipSuccess or an error.
 hipError_t launch_pack_coeffs(int ncomp, const int2 *d_range, const ShapeletCoeff *d_coeffs, int4 *d_prange,
                               PackedCoeff *d_packed, hipStream_t stream);
+// coh_cls[r][c][f] = J_{class_row[r]} S J_{class_row[0]}^H from S = coh ([ncomp][nchan_pad][8]).
+hipError_t launch_class_coherency(int nclass, int ncomp, int nchan, int nchan_pad, const double *d_s,
+                                  const BeamDesc &beam, const int *d_class_row, double *d_coh_cls,
+                                  hipStream_t stream);
 // Component ranges for a baseline subset of nsub baselines (1 when they fill the GPU).
 int direct_subset_split(int nsub, int nchan, int ncomp);
 hipError_t launch_coherency(int ncomp, int nchan, int nchan_pad, const double *d_freqs, FluxBuckets fb,

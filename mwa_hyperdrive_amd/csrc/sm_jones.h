@@ -78,6 +78,22 @@ __device__ __forceinline__ void load_jones(const JonesF64 *p, double j[8]) {
     for (int e = 0; e < 8; ++e) j[e] = q[e];
 }
 
+// A class coherency J_r S J_0^H, or its conjugate transpose J_0 S J_r^H (S is Hermitian)
+// when herm: entry (i, j) from (j, i), conjugated.
+__device__ __forceinline__ void class_coh(const double *p, bool herm, double o[8]) {
+    double v[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = p[e];
+    o[0] = v[0];
+    o[1] = herm ? -v[1] : v[1];
+    o[2] = herm ? v[4] : v[2];
+    o[3] = herm ? -v[5] : v[3];
+    o[4] = herm ? v[2] : v[4];
+    o[5] = herm ? -v[3] : v[5];
+    o[6] = v[6];
+    o[7] = herm ? -v[7] : v[7];
+}
+
 // (A S) B^H in the reference's order (src/model/cpu.rs:343; marlu Jones Mul and h()).
 __device__ __forceinline__ void sandwich(const double a[8], const double s[8], const double b[8], double o[8]) {
     double t[8];

@@ -90,8 +90,8 @@ hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf
     if (e != hipSuccess) return e;
     // Per-tile shapelets with beam-row classes: J_r S J_0^H per class for the channel-per-lane
     // kernel, when the table fits a cap. (Measured on the 256-tile case: that kernel runs 28%
-    // faster. The baseline-per-lane kernels, already at 256 VGPRs, ran 2.2x slower with the
-    // same lookup compiled in, so they keep forming J_p S J_q^H.)
+    // faster. The baseline-per-lane kernels ran 2.2x slower with the same lookup, whether
+    // compiled into them or as a separate instantiation, so they keep forming J_p S J_q^H.)
     BeamDesc beam = ctx.beam;
     const size_t cls_bytes = (size_t)ctx.nclass * n * ctx.nchan_pad * 8 * sizeof(double);
     if (kind == KIND_SHAPELET && g_direct_variant != 1 && ctx.d_bl_subset == nullptr && per_tile &&

@@ -386,9 +386,141 @@ __global__ void k_probe_err(int n_freqs, int n_dirs, int dstride, int fstride, i
 
 using namespace sm;
 
+// ------------------------------------------------------------------ fused FP16 inference
+// The whole network for 64 samples per 4-wave work-group, activations resident in LDS (two
+// 64 x 256 FP16 buffers, so two work-groups share a CU), weights read as MFMA fragments
+// straight from global memory (L2-resident, 16 bytes per lane). Each wave computes 64 neurons
+// of every hidden layer for 64 samples (2 x 2 tiles of v_mfma_f32_32x32x16_f16), then bias,
+// SiLU and the FP16 store into the other buffer; the last layer (8 outputs, padded to 32)
+// gives each of the first waves one 32-sample tile. (Measured on 6.1 M samples: 7.5 ms, against
+// 9.5 ms with 128 samples per 8-wave group and 9.1 ms with 128 per 4-wave group.) The products and their K order are those of the layer-by-layer
+// GEMM; the SiLU uses a fast reciprocal.
+constexpr int kK0 = 64;        // layer-0 K, 53 features padded
+constexpr int kMR = 64;        // samples per work-group (two work-groups per CU)
+constexpr int kMW = kMR / 16;  // waves per work-group: 4 neuron groups x (kMR / 64) sample halves
+constexpr int kST = 2;         // sample tiles per wave
+constexpr int kActPad = 8;     // FP16 row padding: 528-byte rows put 8 rows on distinct banks
+
+struct MlpArgs {
+    const _Float16 *w0;        // [256][64]
+    const _Float16 *w[3];      // layers 1..3, [256][256]
+    const _Float16 *w4;        // [32][256]
+    const float *b[kLayers];
+};
+
+__global__ void k_pad_half(int rows, int cols, int rows_in, int cols_in, const float *__restrict__ src,
+                           _Float16 *__restrict__ dst) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)rows * cols) return;
+    const int r = (int)(i / cols), c = (int)(i % cols);
+    dst[i] = (_Float16)(r < rows_in && c < cols_in ? src[(size_t)r * cols_in + c] : 0.0f);
+}
+
+// SiLU with a fast reciprocal: the network is evaluated in FP16, far coarser than this.
+__device__ __forceinline__ float silu_fast(float z) { return __fdividef(z, 1.0f + __expf(-z)); }
+
+// One hidden layer, computed transposed (out^T = W in^T): the weights are the MFMA A operand
+// and the activations the B operand, so each lane's 4 consecutive accumulator elements are 4
+// consecutive neurons of one sample and leave as one 8-byte LDS store.
+template <int K>
+__device__ __forceinline__ void mlp_hidden(const _Float16 (*in_all)[kHid + kActPad],
+                                           _Float16 (*out_all)[kHid + kActPad], const _Float16 *__restrict__ w,
+                                           const float *__restrict__ bias, int wave, int lane) {
+    // Wave w: neurons 64 (w % 4) .. + 63 for samples 64 (w / 4) .. + 63.
+    const _Float16 (*in)[kHid + kActPad] = in_all + 64 * (wave >> 2);
+    _Float16 (*out)[kHid + kActPad] = out_all + 64 * (wave >> 2);
+    wave &= 3;
+    typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+    f16v acc[2][kST];  // [neuron tile][sample tile]
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+        for (int st = 0; st < kST; ++st)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[nt][st][r] = 0.0f;
+    const int n0 = wave * 64;
+    const int kl = 8 * (lane >> 5);
+#pragma unroll 4
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        h8 wa[2], xb[kST];
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt)
+            wa[nt] = *reinterpret_cast<const h8 *>(&w[(size_t)(n0 + nt * 32 + (lane & 31)) * K + k0 + kl]);
+#pragma unroll
+        for (int st = 0; st < kST; ++st) xb[st] = *reinterpret_cast<const h8 *>(&in[st * 32 + (lane & 31)][k0 + kl]);
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+            for (int st = 0; st < kST; ++st)
+                acc[nt][st] = __builtin_amdgcn_mfma_f32_32x32x16_f16(wa[nt], xb[st], acc[nt][st], 0, 0, 0);
+    }
+    // Element r of lane l: neuron 8 (r / 4) + 4 (l / 32) + r % 4 of the tile, sample l % 32.
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const int nb = n0 + nt * 32 + 8 * g + 4 * (lane >> 5);
+            const float4 bc = *reinterpret_cast<const float4 *>(&bias[nb]);
+#pragma unroll
+            for (int st = 0; st < kST; ++st) {
+                h4 v;
+                v[0] = (_Float16)silu_fast(acc[nt][st][4 * g] + bc.x);
+                v[1] = (_Float16)silu_fast(acc[nt][st][4 * g + 1] + bc.y);
+                v[2] = (_Float16)silu_fast(acc[nt][st][4 * g + 2] + bc.z);
+                v[3] = (_Float16)silu_fast(acc[nt][st][4 * g + 3] + bc.w);
+                *reinterpret_cast<h4 *>(&out[st * 32 + (lane & 31)][nb]) = v;
+            }
+        }
+}
+
+__global__ __launch_bounds__(kMW * 64) void k_mlp16(int n, const float *__restrict__ x, const MlpArgs a,
+                                               float *__restrict__ y) {
+    __shared__ __attribute__((aligned(16))) _Float16 act[2][kMR][kHid + kActPad];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const size_t row0 = (size_t)blockIdx.x * kMR;
+    // Features in FP16, K padded with zeros to 64; rows past n are zeros.
+    for (int e = tid; e < kMR * kK0; e += kMW * 64) {
+        const int r = e / kK0, k = e % kK0;
+        const size_t gr = row0 + r;
+        act[0][r][k] = (_Float16)(gr < (size_t)n && k < kNIn ? x[gr * kNIn + k] : 0.0f);
+    }
+    __syncthreads();
+    mlp_hidden<kK0>(act[0], act[1], a.w0, a.b[0], wave, lane);
+    __syncthreads();
+    mlp_hidden<kHid>(act[1], act[0], a.w[0], a.b[1], wave, lane);
+    __syncthreads();
+    mlp_hidden<kHid>(act[0], act[1], a.w[1], a.b[2], wave, lane);
+    __syncthreads();
+    mlp_hidden<kHid>(act[1], act[0], a.w[2], a.b[3], wave, lane);
+    __syncthreads();
+    // Output layer: wave w < 4 takes samples 32 w .. + 31; columns 0..7 of the padded 32 are
+    // real.
+    if (wave >= kMR / 32) return;
+    f16v acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+    const int kl = 8 * (lane >> 5);
+#pragma unroll 4
+    for (int k0 = 0; k0 < kHid; k0 += 16) {
+        const h8 b = *reinterpret_cast<const h8 *>(&a.w4[(size_t)(lane & 31) * kHid + k0 + kl]);
+        const h8 av = *reinterpret_cast<const h8 *>(&act[0][wave * 32 + (lane & 31)][k0 + kl]);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, b, acc, 0, 0, 0);
+    }
+    const int col = lane & 31;
+    if (col < kNOut) {
+        const float bc = a.b[kLayers - 1][col];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const size_t gr = row0 + wave * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            if (gr < (size_t)n) y[gr * kNOut + col] = acc[r] + bc;
+        }
+    }
+}
+
 struct SmSurrogate {
     DevBuf w[kLayers], b[kLayers], mw[kLayers], vw[kLayers], mb[kLayers], vb[kLayers], gw[kLayers], gb[kLayers];
     DevBuf w16[kLayers];
+    DevBuf w0p, w4p;  // FP16 layer 0 padded to K = 64 and layer 4 padded to 32 outputs
     bool w16_valid = false;
     DevBuf z[kLayers], h[kLayers];  // training activations: z pre-activation, h post (h[0] = input)
     DevBuf dz, dh, dz2, sse;
@@ -408,6 +540,12 @@ const char *ensure_w16(SmSurrogate *s) {
         hipLaunchKernelGGL(k_to_half, dim3(blocks(wsize(l))), dim3(256), 0, 0, wsize(l), s->w[l].as<float>(),
                            s->w16[l].as<_Float16>());
     }
+    SM_TRY(s->w0p.reserve(sizeof(_Float16) * kHid * kK0));
+    SM_TRY(s->w4p.reserve(sizeof(_Float16) * 32 * kHid));
+    hipLaunchKernelGGL(k_pad_half, dim3(blocks((size_t)kHid * kK0)), dim3(256), 0, 0, kHid, kK0, kHid, kNIn,
+                       s->w[0].as<float>(), s->w0p.as<_Float16>());
+    hipLaunchKernelGGL(k_pad_half, dim3(blocks((size_t)32 * kHid)), dim3(256), 0, 0, 32, kHid, kNOut, kHid,
+                       s->w[kLayers - 1].as<float>(), s->w4p.as<_Float16>());
     SM_TRY(hipGetLastError());
     s->w16_valid = true;
     return nullptr;
@@ -470,21 +608,14 @@ const char *backward32(SmSurrogate *s, int n, const float *d_x, const float *d_y
 // FP16 MFMA forward over n rows of FP32 features; FP32 outputs [n][8].
 const char *forward16(SmSurrogate *s, int n, const float *d_x, float *d_y) {
     if (const char *e = ensure_w16(s)) return e;
-    for (int i = 0; i < 2; ++i) SM_TRY(s->a16[i].reserve(sizeof(_Float16) * (size_t)n * kHid));
-    // Layer 0 reads FP32 features and converts while staging; later layers read FP16.
-    SM_TRY((gemm<_Float16, float, false, true>(n, kHid, kNIn, d_x, kNIn, s->w[0].as<float>(), kNIn, nullptr, kHid,
-                                               EPI_BIAS_SILU_HALF, s->b[0].as<float>(), s->a16[0].ptr, 0)));
-    int cur = 0;
-    for (int l = 1; l < kLayers - 1; ++l) {
-        SM_TRY((gemm<_Float16, _Float16, false, true>(n, kHid, kHid, s->a16[cur].as<_Float16>(), kHid,
-                                                      s->w16[l].as<_Float16>(), kHid, nullptr, kHid,
-                                                      EPI_BIAS_SILU_HALF, s->b[l].as<float>(), s->a16[1 - cur].ptr,
-                                                      0)));
-        cur = 1 - cur;
-    }
-    SM_TRY((gemm<_Float16, _Float16, false, true>(n, kNOut, kHid, s->a16[cur].as<_Float16>(), kHid,
-                                                  s->w16[kLayers - 1].as<_Float16>(), kHid, d_y, kNOut, EPI_BIAS,
-                                                  s->b[kLayers - 1].as<float>(), nullptr, 0)));
+    MlpArgs a;
+    a.w0 = s->w0p.as<_Float16>();
+    for (int l = 1; l < kLayers - 1; ++l) a.w[l - 1] = s->w16[l].as<_Float16>();
+    a.w4 = s->w4p.as<_Float16>();
+    for (int l = 0; l < kLayers; ++l) a.b[l] = s->b[l].as<float>();
+    const unsigned grid = (unsigned)(((size_t)n + kMR - 1) / kMR);
+    hipLaunchKernelGGL(k_mlp16, dim3(grid), dim3(kMW * 64), 0, 0, n, d_x, a, d_y);
+    SM_TRY(hipGetLastError());
     return nullptr;
 }
 

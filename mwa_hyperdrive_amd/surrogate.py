@@ -49,7 +49,10 @@ class BeamSurrogate:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
-            _lib.lib().sm_surrogate_free(h)
+            try:
+                _lib.lib().sm_surrogate_free(h)
+            except Exception:  # interpreter shutdown: the library may already be gone
+                pass
             self._h = None
 
     @staticmethod

@@ -138,6 +138,46 @@ hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf
     a.vis64 = vis64;
     a.bl_subset = ctx.d_bl_subset;
     a.nsub = ctx.nsub;
+    const bool g0_tiled = per_tile && ctx.d_bl_subset == nullptr && g_direct_variant == 0 && kind != KIND_SHAPELET &&
+                          ctx.even && ctx.g0_n >= 2 && ctx.d_g0_uvw != nullptr && ctx.class_row0 >= 0;
+    if (g0_tiled) {
+        // Pairs of two class-0 tiles: J_0 S J_0^H, shared by all of them, on the tile-pair
+        // kernel over the class-0 tiles; the other baselines: per-tile kernel on a subset.
+        thread_local DevBuf coh0;
+        e = coh0.reserve((size_t)n * ctx.nchan_pad * 8 * sizeof(double));
+        if (e != hipSuccess) return e;
+        BeamDesc b0 = ctx.beam;
+        b0.uniform_row = ctx.class_row0;
+        e = launch_coherency(n, ctx.nchan, ctx.nchan_pad, ctx.d_freqs_pad, in.fb, b0, true, coh0.as<double>(), stream);
+        if (e != hipSuccess) return e;
+        TiledArgs t;
+        t.ntiles = ctx.g0_n;
+        t.nbl = ctx.nbl;
+        t.nchan = ctx.nchan;
+        t.nchan_pad = ctx.nchan_pad;
+        t.ncomp = n;
+        t.npairs = ctx.g0_npairs;
+        t.nchunks = (ctx.nchan + SM_NCH - 1) / SM_NCH;
+        t.pairs = ctx.d_g0_pairs;
+        t.freqs = ctx.d_freqs_pad;
+        t.tile_uvw = ctx.d_g0_uvw;
+        t.lmn = in.lmn;
+        t.shape = in.shape;
+        t.coh = coh0.as<double>();
+        t.vis32 = vis32;
+        t.vis64 = vis64;
+        t.tile_map = ctx.d_g0_tiles;
+        t.ntiles_bl = ctx.ntiles;
+        if (ev0 != nullptr) (void)hipEventRecord(ev0, stream);
+        e = launch_direct_tiled(kind, t, stream);
+        if (e == hipSuccess && ctx.n_notg0 > 0) {
+            a.bl_subset = ctx.d_notg0_bl;
+            a.nsub = ctx.n_notg0;
+            e = launch_direct(kind, per_tile, ctx.even, a, stream);
+        }
+        if (ev1 != nullptr) (void)hipEventRecord(ev1, stream);
+        return e != hipSuccess ? e : hipGetLastError();
+    }
     if (ev0 != nullptr) (void)hipEventRecord(ev0, stream);
     if (ctx.d_bl_subset != nullptr) {
         a.csplit = direct_subset_split(ctx.nsub, ctx.nchan, n);

@@ -150,7 +150,12 @@ __global__ __launch_bounds__(256) void k_direct_tiled(const TiledArgs a) {
         __syncthreads();
     }
     if (!valid) return;
-    const size_t bl = (size_t)p * a.ntiles - (size_t)p * (p + 1) / 2 + (size_t)(q - p - 1);
+    if (a.tile_map != nullptr) {
+        p = a.tile_map[p];
+        q = a.tile_map[q];
+    }
+    const size_t nt = a.tile_map != nullptr ? (size_t)a.ntiles_bl : (size_t)a.ntiles;
+    const size_t bl = (size_t)p * nt - (size_t)p * (p + 1) / 2 + (size_t)(q - p - 1);
 #pragma unroll
     for (int k = 0; k < SM_NCH; ++k) {
         const int f = f0 + k;

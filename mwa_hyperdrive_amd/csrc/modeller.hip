@@ -224,6 +224,10 @@ struct SmModeller {
     int g0_row = -1, g0_n = 0, g0_nblocks = 0, g0_nmap = 0, g0_nrest = 0, g0_nclass = 1;
     DevBuf d_g0_tiles, d_g0_blocks, d_g0_block_of, d_g0_map, d_g0_rest, d_g0_uvw, d_g0_coh;
     DevBuf d_g0_rclass, d_g0_class_row, d_class_of_row;
+    // Class-0 tiles for the tile-pair kernel: their tile-group pairs, gathered UVWs and the
+    // baselines with a tile outside class 0.
+    int g0_n0 = 0, g0_npairs = 0, n_notg0 = 0;
+    DevBuf d_g0_pairs, d_g0_only_uvw, d_notg0_bl;
     int npairs = 0;
     std::vector<double> h_tile_uvw;
     DevType types[3];
@@ -598,6 +602,17 @@ extern "C" const char *sm_new(const SmObsParams *p, const SmComponent *comps, co
                         block_of[(size_t)p * G + q] = (int)blocks.size();
                         blocks.push_back(make_int2(p, q));
                     }
+                {
+                    const std::vector<int4> p0 = tile_pairs(n0);
+                    std::vector<int> notg0;
+                    for (int b = 0; b < m->nbl; ++b)
+                        if (m->tile_rows[bl_tiles[b].x] != r0 || m->tile_rows[bl_tiles[b].y] != r0) notg0.push_back(b);
+                    m->g0_n0 = n0;
+                    m->g0_npairs = (int)p0.size();
+                    m->n_notg0 = (int)notg0.size();
+                    SM_NEW_TRY(upload(m->d_g0_pairs, p0));
+                    SM_NEW_TRY(upload(m->d_notg0_bl, notg0));
+                }
                 std::vector<int4> map;
                 std::vector<int> rest;
                 for (int b = 0; b < m->nbl; ++b) {
@@ -711,6 +726,18 @@ static const char *model_timestep_device(SmModeller *m, double lst, double latit
         ctx.class_row0 = m->g0_row;
         ctx.d_class_row = m->d_g0_class_row.as<int>();
         ctx.d_class_of_row = m->d_class_of_row.as<int>();
+        if (tile_uvw != nullptr && m->g0_n0 >= 2) {
+            // The first g0_n0 factored tiles are the class-0 tiles, ascending.
+            SM_TRY(m->d_g0_only_uvw.reserve(sizeof(double) * 3 * m->g0_n0));
+            SM_TRY(launch_gather_rows3(m->g0_n0, m->d_g0_tiles.as<int>(), tile_uvw, m->d_g0_only_uvw.as<double>(), 0));
+            ctx.g0_n = m->g0_n0;
+            ctx.g0_npairs = m->g0_npairs;
+            ctx.n_notg0 = m->n_notg0;
+            ctx.d_g0_tiles = m->d_g0_tiles.as<int>();
+            ctx.d_g0_uvw = m->d_g0_only_uvw.as<double>();
+            ctx.d_g0_pairs = m->d_g0_pairs.as<int4>();
+            ctx.d_notg0_bl = m->d_notg0_bl.as<int>();
+        }
     }
     const Kind kinds[3] = {KIND_POINT, KIND_GAUSSIAN, KIND_SHAPELET};
     for (int ty = 0; ty < 3; ++ty) {

@@ -33,6 +33,13 @@ struct ModelCtx {
     // Per-tile beams, optional: beam-row classes (class 0 = row class_row[0]); the per-tile
     // kernels then read J_r S J_0^H from a class table for baselines with a class-0 tile.
     int nclass = 0, class_row0 = -1;     // class_row0 = the beam row of class 0
+    // Per-tile beams, optional: pairs of two class-0 tiles share J_0 S J_0^H and take the
+    // tile-pair kernel on the class-0 tiles; every other baseline takes the per-tile kernel.
+    int g0_n = 0, g0_npairs = 0, n_notg0 = 0;
+    const int *d_g0_tiles = nullptr;    // [g0_n] class-0 array tiles, ascending
+    const double *d_g0_uvw = nullptr;   // [g0_n][3] their tile UVWs
+    const int4 *d_g0_pairs = nullptr;   // tile-group pairs over the class-0 tiles
+    const int *d_notg0_bl = nullptr;    // [n_notg0] baselines with a tile outside class 0
     const int *d_class_row = nullptr;   // [nclass] beam row of each class
     const int *d_class_of_row = nullptr;  // [n_utile]
 };

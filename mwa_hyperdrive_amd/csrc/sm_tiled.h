@@ -18,6 +18,10 @@ struct TiledArgs {
     const double *coh;        // [ncomp][nchan_pad][8]
     JonesF32 *vis32;
     JonesF64 *vis64;
+    // Optional tile subset: kernel tile i is array tile tile_map[i] (ascending), and output
+    // baselines are numbered over ntiles_bl array tiles.
+    const int *tile_map = nullptr;
+    int ntiles_bl = 0;
 };
 
 struct ShapeletChanArgs {

@@ -78,6 +78,19 @@ double4 host_gaussian_shape(GaussianParams g) { return gaussian_shape(g); }
 double4 host_shapelet_shape(GaussianParams g) { return shapelet_shape(g); }
 
 // ------------------------------------------------------------------ shared driver
+// The baseline kernel on the subset a.bl_subset / a.nsub, splitting the component loop when
+// the subset fills too few lanes (one FP64 accumulation buffer per thread of the caller).
+static hipError_t launch_subset(Kind kind, bool per_tile, bool even, DirectArgs a, hipStream_t stream) {
+    a.csplit = direct_subset_split(a.nsub, a.nchan, a.ncomp);
+    if (a.csplit > 1) {
+        thread_local DevBuf accum;
+        hipError_t e = accum.reserve(sizeof(double) * 8 * (size_t)a.nchan * a.nsub);
+        if (e != hipSuccess) return e;
+        a.accum = accum.as<double>();
+    }
+    return launch_direct(kind, per_tile, even, a, stream);
+}
+
 hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf &coh, JonesF32 *vis32,
                     JonesF64 *vis64, hipStream_t stream, hipEvent_t ev0, hipEvent_t ev1) {
     const int n = in.fb.n_pl + in.fb.n_cpl + in.fb.n_list;
@@ -170,24 +183,41 @@ hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf
         t.ntiles_bl = ctx.ntiles;
         if (ev0 != nullptr) (void)hipEventRecord(ev0, stream);
         e = launch_direct_tiled(kind, t, stream);
-        if (e == hipSuccess && ctx.n_notg0 > 0) {
-            a.bl_subset = ctx.d_notg0_bl;
-            a.nsub = ctx.n_notg0;
-            e = launch_direct(kind, per_tile, ctx.even, a, stream);
+        // Pairs of a class-r tile with a class-0 tile: one shared coherency per segment, so
+        // the shared-beam form of the baseline kernel (no per-visibility Jones products).
+        thread_local DevBuf cseg;
+        const int nseg = 2 * (ctx.nclass - 1);
+        for (int sgi = 0; sgi < nseg && e == hipSuccess; ++sgi) {
+            const int off = ctx.h_seg_off[sgi], cnt = ctx.h_seg_off[sgi + 1] - off;
+            if (cnt <= 0) continue;
+            const int r = 1 + sgi / 2;
+            const bool fwd = (sgi & 1) == 0;
+            e = cseg.reserve((size_t)n * ctx.nchan_pad * 8 * sizeof(double));
+            if (e != hipSuccess) break;
+            const int ra = fwd ? ctx.h_class_row[r] : ctx.class_row0, rb = fwd ? ctx.class_row0 : ctx.h_class_row[r];
+            e = launch_pair_coherency(n, ctx.nchan, ctx.nchan_pad, coh.as<double>(), ctx.beam, ra, rb,
+                                      cseg.as<double>(), stream);
+            if (e != hipSuccess) break;
+            DirectArgs u = a;
+            u.coh = cseg.as<double>();
+            u.bl_subset = ctx.d_notg0_bl + off;
+            u.nsub = cnt;
+            e = launch_subset(kind, false, ctx.even, u, stream);
+        }
+        // Pairs of two tiles outside class 0: per-tile J_p S J_q^H.
+        const int off = ctx.h_seg_off[nseg], cnt = ctx.h_seg_off[nseg + 1] - off;
+        if (e == hipSuccess && cnt > 0) {
+            DirectArgs u = a;
+            u.bl_subset = ctx.d_notg0_bl + off;
+            u.nsub = cnt;
+            e = launch_subset(kind, true, ctx.even, u, stream);
         }
         if (ev1 != nullptr) (void)hipEventRecord(ev1, stream);
         return e != hipSuccess ? e : hipGetLastError();
     }
     if (ev0 != nullptr) (void)hipEventRecord(ev0, stream);
     if (ctx.d_bl_subset != nullptr) {
-        a.csplit = direct_subset_split(ctx.nsub, ctx.nchan, n);
-        if (a.csplit > 1) {
-            thread_local DevBuf accum;
-            e = accum.reserve(sizeof(double) * 8 * (size_t)ctx.nchan * ctx.nsub);
-            if (e != hipSuccess) return e;
-            a.accum = accum.as<double>();
-        }
-        e = launch_direct(kind, per_tile, ctx.even, a, stream);
+        e = launch_subset(kind, per_tile, ctx.even, a, stream);
     } else if (g_direct_variant == 0 && kind != KIND_SHAPELET && !per_tile && ctx.even && ctx.d_tile_uvw != nullptr &&
         ctx.d_pairs != nullptr && ctx.ntiles >= 2) {
         TiledArgs t;

@@ -17,6 +17,7 @@
 
 #include "sm_internal.h"
 #include "sm_jones.h"
+#include "sm_math.h"
 
 namespace sm {
 
@@ -90,6 +91,37 @@ __global__ void k_class_coherency(int nclass, int ncomp, int nchan, int nchan_pa
     sandwich(ja, sv, jb, v);
 #pragma unroll
     for (int e = 0; e < 8; ++e) o[e] = v[e];
+}
+
+__global__ void k_pair_coherency(int ncomp, int nchan, int nchan_pad, const double *__restrict__ s_tab, BeamDesc beam,
+                                 int row_a, int row_b, double *__restrict__ out) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)ncomp * nchan_pad) return;
+    const int f = (int)(idx % nchan_pad);
+    const int c = (int)(idx / nchan_pad);
+    double *o = out + idx * 8;
+    if (f >= nchan) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) o[e] = 0.0;
+        return;
+    }
+    double sv[8], ja[8], jb[8], v[8];
+    load_jones(reinterpret_cast<const JonesF64 *>(s_tab + idx * 8), sv);
+    const size_t col = (size_t)beam.freq_map[f] * beam.n_dir + beam.dir_offset + c;
+    load_jones(&beam.jones[(size_t)row_a * beam.n_ufreq * beam.n_dir + col], ja);
+    load_jones(&beam.jones[(size_t)row_b * beam.n_ufreq * beam.n_dir + col], jb);
+    sandwich(ja, sv, jb, v);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) o[e] = v[e];
+}
+
+hipError_t launch_pair_coherency(int ncomp, int nchan, int nchan_pad, const double *d_s, const BeamDesc &beam,
+                                 int row_a, int row_b, double *d_out, hipStream_t stream) {
+    const size_t n = (size_t)ncomp * nchan_pad;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pair_coherency, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, ncomp, nchan,
+                       nchan_pad, d_s, beam, row_a, row_b, d_out);
+    return hipGetLastError();
 }
 
 hipError_t launch_class_coherency(int nclass, int ncomp, int nchan, int nchan_pad, const double *d_s,
@@ -225,8 +257,8 @@ __global__ __launch_bounds__(256) void k_direct(const DirectArgs a, int gx, int 
         const double theta = uvw.u * lmn.x + uvw.v * lmn.y + uvw.w * lmn.z;
         double zr = 1.0, zi = 0.0, rr = 1.0, ri = 0.0;
         if (EVEN) {
-            sincos(theta * f0c, &zi, &zr);
-            sincos(theta * dfc, &ri, &rr);
+            fast_sincos(theta * f0c, zi, zr);
+            fast_sincos(theta * dfc, ri, rr);
         }
         double q = 0.0, env = 1.0, er = 1.0, eg = 1.0;
         double xk = 0.0, yk = 0.0;
@@ -254,7 +286,7 @@ __global__ __launch_bounds__(256) void k_direct(const DirectArgs a, int gx, int 
                 pr = zr;
                 pi = zi;
             } else {
-                sincos(theta * (freqs[f0 + k] / SM_VEL_C), &pi, &pr);
+                fast_sincos(theta * (freqs[f0 + k] / SM_VEL_C), pi, pr);
             }
             if (KIND == KIND_GAUSSIAN) {
                 const double e = EVEN ? env : exp(q * (freqs[f0 + k] * freqs[f0 + k]));

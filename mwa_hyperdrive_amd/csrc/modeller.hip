@@ -226,8 +226,9 @@ struct SmModeller {
     DevBuf d_g0_rclass, d_g0_class_row, d_class_of_row;
     // Class-0 tiles for the tile-pair kernel: their tile-group pairs, gathered UVWs and the
     // baselines with a tile outside class 0.
-    int g0_n0 = 0, g0_npairs = 0, n_notg0 = 0;
+    int g0_n0 = 0, g0_npairs = 0;
     DevBuf d_g0_pairs, d_g0_only_uvw, d_notg0_bl;
+    std::vector<int> h_seg_off, h_class_row;  // segments of d_notg0_bl; beam row per class
     int npairs = 0;
     std::vector<double> h_tile_uvw;
     DevType types[3];
@@ -603,13 +604,31 @@ extern "C" const char *sm_new(const SmObsParams *p, const SmComponent *comps, co
                         blocks.push_back(make_int2(p, q));
                     }
                 {
+                    // Baselines with a tile outside class 0, by segment: (class r, class 0),
+                    // (class 0, class r) for r = 1 .. R-1, then (other, other).
                     const std::vector<int4> p0 = tile_pairs(n0);
+                    const int R = (int)class_row.size();
+                    std::vector<std::vector<int>> seg(2 * (R - 1) + 1);
+                    for (int b = 0; b < m->nbl; ++b) {
+                        const int ci = class_of_row[m->tile_rows[bl_tiles[b].x]];
+                        const int cj = class_of_row[m->tile_rows[bl_tiles[b].y]];
+                        if (ci == 0 && cj == 0) continue;
+                        if (cj == 0)
+                            seg[2 * (ci - 1)].push_back(b);
+                        else if (ci == 0)
+                            seg[2 * (cj - 1) + 1].push_back(b);
+                        else
+                            seg[2 * (R - 1)].push_back(b);
+                    }
                     std::vector<int> notg0;
-                    for (int b = 0; b < m->nbl; ++b)
-                        if (m->tile_rows[bl_tiles[b].x] != r0 || m->tile_rows[bl_tiles[b].y] != r0) notg0.push_back(b);
+                    m->h_seg_off.assign(1, 0);
+                    for (const auto &sg : seg) {
+                        notg0.insert(notg0.end(), sg.begin(), sg.end());
+                        m->h_seg_off.push_back((int)notg0.size());
+                    }
+                    m->h_class_row = class_row;
                     m->g0_n0 = n0;
                     m->g0_npairs = (int)p0.size();
-                    m->n_notg0 = (int)notg0.size();
                     SM_NEW_TRY(upload(m->d_g0_pairs, p0));
                     SM_NEW_TRY(upload(m->d_notg0_bl, notg0));
                 }
@@ -732,7 +751,8 @@ static const char *model_timestep_device(SmModeller *m, double lst, double latit
             SM_TRY(launch_gather_rows3(m->g0_n0, m->d_g0_tiles.as<int>(), tile_uvw, m->d_g0_only_uvw.as<double>(), 0));
             ctx.g0_n = m->g0_n0;
             ctx.g0_npairs = m->g0_npairs;
-            ctx.n_notg0 = m->n_notg0;
+            ctx.h_seg_off = m->h_seg_off.data();
+            ctx.h_class_row = m->h_class_row.data();
             ctx.d_g0_tiles = m->d_g0_tiles.as<int>();
             ctx.d_g0_uvw = m->d_g0_only_uvw.as<double>();
             ctx.d_g0_pairs = m->d_g0_pairs.as<int4>();

@@ -35,11 +35,18 @@ struct ModelCtx {
     int nclass = 0, class_row0 = -1;     // class_row0 = the beam row of class 0
     // Per-tile beams, optional: pairs of two class-0 tiles share J_0 S J_0^H and take the
     // tile-pair kernel on the class-0 tiles; every other baseline takes the per-tile kernel.
-    int g0_n = 0, g0_npairs = 0, n_notg0 = 0;
+    // Baselines with a tile outside class 0, in segments: for each class r = 1 .. nclass-1,
+    // first those whose first tile is in class r and second in class 0 (coherency
+    // J_r S J_0^H, shared by the segment), then the reverse (J_0 S J_r^H); last, the pairs of
+    // two tiles outside class 0 (per-tile J_p S J_q^H). h_seg_off (host) holds the
+    // 2 (nclass - 1) + 2 segment offsets into d_notg0_bl, h_class_row the beam row per class.
+    int g0_n = 0, g0_npairs = 0;
     const int *d_g0_tiles = nullptr;    // [g0_n] class-0 array tiles, ascending
     const double *d_g0_uvw = nullptr;   // [g0_n][3] their tile UVWs
     const int4 *d_g0_pairs = nullptr;   // tile-group pairs over the class-0 tiles
-    const int *d_notg0_bl = nullptr;    // [n_notg0] baselines with a tile outside class 0
+    const int *d_notg0_bl = nullptr;
+    const int *h_seg_off = nullptr;
+    const int *h_class_row = nullptr;
     const int *d_class_row = nullptr;   // [nclass] beam row of each class
     const int *d_class_of_row = nullptr;  // [n_utile]
 };

@@ -94,6 +94,9 @@ hipError_t launch_pack_coeffs(int ncomp, const int2 *d_range, const ShapeletCoef
 hipError_t launch_class_coherency(int nclass, int ncomp, int nchan, int nchan_pad, const double *d_s,
                                   const BeamDesc &beam, const int *d_class_row, double *d_coh_cls,
                                   hipStream_t stream);
+// out[c][f] = J_{row_a} S J_{row_b}^H from S = d_s ([ncomp][nchan_pad][8]).
+hipError_t launch_pair_coherency(int ncomp, int nchan, int nchan_pad, const double *d_s, const BeamDesc &beam,
+                                 int row_a, int row_b, double *d_out, hipStream_t stream);
 // Component ranges for a baseline subset of nsub baselines (1 when they fill the GPU).
 int direct_subset_split(int nsub, int nchan, int ncomp);
 hipError_t launch_coherency(int ncomp, int nchan, int nchan_pad, const double *d_freqs, FluxBuckets fb,

@@ -406,9 +406,9 @@ __global__ __launch_bounds__(256, 2) void k_factored(const FactoredArgs a) {
 
 // ---------------------------------------------------------------- wave-specialised variant
 // Four consumer waves (one per SIMD) run only the MFMAs of step s from one LDS buffer while
-// 4 or 8 producer waves generate step s + 1 into the other; one barrier per step. Each SIMD
-// thus always holds a wave with MFMAs to issue and one with ALU work, instead of relying on
-// two general waves to drift out of phase. Register use is the larger of the two roles.
+// 8 producer waves generate step s + 1 into the other; one barrier per step. Each SIMD thus
+// always holds a wave with MFMAs to issue and two with ALU work, instead of relying on two
+// general waves to drift out of phase. Register use is the larger of the two roles.
 
 // A producer thread's operands for one step: 2 components.
 struct Ops2 {

@@ -113,3 +113,40 @@ def test_factored_reused_across_sky_sizes():
         err = rel_err(d64.cpu().numpy(), o64)
         print(f"call {step}: {m.info()['num_points']} points, rel err {err:.3e}")
         assert err <= 1e-6
+
+
+def test_per_tile_beam_classes_fp64_and_split_vs_oracle():
+    """Per-tile beams with several beam rows, as in the combined 256-antenna case: 40 tiles,
+    7 with a dead dipole, two pairs of them sharing the same dead dipole (one beam row each),
+    all three component types. The FP64 path splits the baselines by beam-row class: pairs of
+    two common-row tiles on the tile-pair kernel, pairs of a class-r tile and a common-row
+    tile on the shared-coherency baseline kernel (both orientations), the remaining pairs with
+    per-visibility J_p S J_q^H, and the shapelets through the per-class coherency table. Every
+    baseline must meet the FP64 bar; the split-precision path the fp16 bar."""
+    base = synthetic.shrink(synthetic.config(2, scale=120 / 8000.0), tiles_n=40, chans=11, timesteps=1)
+    n = 40
+    delays = np.zeros((n, 16), np.uint32)
+    gains = np.ones((n, 32))
+    for t, d in ((2, 5), (9, 5), (17, 11), (30, 11), (35, 0)):
+        delays[t, d] = 32
+    gains[12, 3] = 0.0
+    gains[21, 40 % 32] = 0.0
+    m = SkyModeller(AnalyticBeam(delays, gains), base.srclist, 0, base.xyzs, base.freqs, [], base.phase_centre, 0.0,
+                    base.latitude)
+    assert m.info()["num_unique_tiles"] == 6
+    o = orc.Obs(base.xyzs, base.freqs, base.phase_centre, float(base.lsts[0]), base.latitude, beam="analytic",
+                delays=delays, gains=gains)
+    o64 = orc.model_timestep(o, base.srclist)[1]
+    _, _, v64 = m.model_timestep(float(base.lsts[0]), base.latitude, want_f64=True)
+    err = rel_err(v64, o64)
+    print(f"per-tile classes, FP64 direct: rel err {err:.3e}")
+    assert err <= 1e-12
+    shape = (len(base.freqs), n * (n - 1) // 2, 8)
+    d32 = torch.zeros(shape, dtype=torch.float32, device="cuda")
+    d64 = torch.zeros(shape, dtype=torch.float64, device="cuda")
+    _lib.call("sm_model_timestep_factored", m._h, float(base.lsts[0]), base.latitude, 0,
+              C.c_void_p(d32.data_ptr()), C.c_void_p(d64.data_ptr()))
+    torch.cuda.synchronize()
+    err = rel_err(d64.cpu().numpy(), o64)
+    print(f"per-tile classes, split precision fp16: rel err {err:.3e}")
+    assert err <= 1e-6

@@ -271,10 +271,35 @@ __global__ __launch_bounds__(256) void k_shapelet_chan(const ShapeletChanArgs a)
             // Stage rows 0 .. pr4.w - 1 of both windows in this wave's LDS, then interpolate
             // from LDS: one 16-byte read per axis per coefficient instead of two gathers.
             const int bx0 = __builtin_amdgcn_readfirstlane(ix0), by0 = __builtin_amdgcn_readfirstlane(iy0);
+            // All loads first, then all LDS writes: the loads are in flight together instead of
+            // one round trip per row. A lane past the window or past row pr4.w - 1 reads an
+            // in-bounds cell (the window's last cell; rows below kRows < SM_SBF_N) that is never
+            // written. Byte offsets are 32-bit (the table is 8 MB), from the uniform base.
             const int j = lane & (kWin - 1), half = lane / kWin;
-            for (int r = half; r < pr4.w; r += 64 / kWin) {
-                if (j < xw) wx_lds[wv][r][j] = sbf[(size_t)r * SM_SBF_L + bx0 + j];
-                if (j < yw) wy_lds[wv][r][j] = sbf[(size_t)r * SM_SBF_L + by0 + j];
+            const int jxc = min(j, xw - 1), jyc = min(j, yw - 1);
+            static_assert(kRows <= SM_SBF_N, "staged rows must exist in the basis table");
+            const unsigned ox = (unsigned)((half * SM_SBF_L + bx0 + jxc) * 8);
+            const unsigned oy = (unsigned)((half * SM_SBF_L + by0 + jyc) * 8);
+            const char *sbf_b = reinterpret_cast<const char *>(sbf);
+            constexpr int kBatch = 2;  // row steps in flight at once (register budget)
+            for (int i0 = 0; i0 * (64 / kWin) < pr4.w; i0 += kBatch) {
+                double tx[kBatch], ty[kBatch];
+#pragma unroll
+                for (int i = 0; i < kBatch; ++i) {
+                    const unsigned step = (unsigned)((i0 + i) * (64 / kWin) * SM_SBF_L * 8);
+                    if ((i0 + i) * (64 / kWin) < pr4.w) {
+                        tx[i] = *reinterpret_cast<const double *>(sbf_b + (ox + step));
+                        ty[i] = *reinterpret_cast<const double *>(sbf_b + (oy + step));
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < kBatch; ++i) {
+                    const int r = half + (i0 + i) * (64 / kWin);
+                    if ((i0 + i) * (64 / kWin) < pr4.w && r < pr4.w) {
+                        if (j < xw) wx_lds[wv][r][j] = tx[i];
+                        if (j < yw) wy_lds[wv][r][j] = ty[i];
+                    }
+                }
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -286,6 +311,7 @@ __global__ __launch_bounds__(256) void k_shapelet_chan(const ShapeletChanArgs a)
                 const PackedCoeff *cp = packed + pr4.x + (pt ? pr4.y : 0);
                 const int cnt = pt ? pr4.z : pr4.y;
                 double accp = 0.0;
+#pragma unroll 2
                 for (int k = 0; k < cnt; ++k) {
                     double val;
                     int n1, n2;
@@ -305,7 +331,28 @@ __global__ __launch_bounds__(256) void k_shapelet_chan(const ShapeletChanArgs a)
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         } else {
-            shapelet_sum(packed, pr4, sbf, ix, iy, wx, wy, er, ei);
+            // Rare (a window wider than the LDS copy): one coefficient at a time, which keeps
+            // this path's registers below the staged path's.
+            double part[2];
+#pragma unroll
+            for (int pt = 0; pt < 2; ++pt) {
+                const PackedCoeff *cp = packed + pr4.x + (pt ? pr4.y : 0);
+                const int cnt = pt ? pr4.z : pr4.y;
+                double accp = 0.0;
+#pragma unroll 1
+                for (int k = 0; k < cnt; ++k) {
+                    double val;
+                    int n1, n2;
+                    load_packed_uniform(cp + k, val, n1, n2);
+                    const double *bx = sbf + n1 * SM_SBF_L + ix, *by = sbf + n2 * SM_SBF_L + iy;
+                    const double u_value = bx[0] + (bx[1] - bx[0]) * wx;
+                    const double v_value = by[0] + (by[1] - by[0]) * wy;
+                    accp += (val * u_value) * v_value;
+                }
+                part[pt] = accp;
+            }
+            er = part[0];
+            ei = part[1];
         }
         const double pr = zr * er - zi * ei;
         const double pi = zr * ei + zi * er;

@@ -2,7 +2,7 @@
 // beam) and evenly spaced channels.
 //
 // k_direct_tiled (points, Gaussians): a block owns a 16 x 16 block of tile pairs and one
-// chunk of SM_NCH channels. The baseline phase factorises over antennas,
+// chunk of NCH channels (kTiledNchPoint, kTiledNchGaussian). The baseline phase factorises over antennas,
 //   e^{i theta_pq f/c} = e^{i theta_p f/c} e^{-i theta_q f/c},  theta_t = u_t L + v_t M + w_t N,
 // (baseline uvw = tile1 - tile2, mdbook/src/defs/coords.md), so per component the block
 // evaluates the start phasor and the per-channel step for its 32 tiles into LDS (64
@@ -25,8 +25,16 @@ constexpr int kTG = 16;   // tiles per group
 constexpr int kWin = 32;  // shapelet table window per wave and axis, cells
 constexpr int kRows = 16; // shapelet table rows staged per wave
 constexpr int kCT = 32;   // components per LDS stage
+// Channels per lane in k_direct_tiled. Fewer channels hold fewer accumulators (4 waves per
+// SIMD instead of 3 for points) but share each component's phasor products over fewer
+// channels. Measured on C2 (768 channels): points 35.1 ms with 8, 33.6 ms with 4; Gaussians
+// 19.7 ms with 8, 21.7 ms with 4.
+constexpr int kTiledNchPoint = 4;
+constexpr int kTiledNchGaussian = 8;
+static_assert(kTiledNchPoint <= SM_NCH && kTiledNchGaussian <= SM_NCH,
+              "the padded channel count covers whole chunks of SM_NCH channels");
 
-template <int KIND>
+template <int KIND, int NCH>
 __global__ __launch_bounds__(256) void k_direct_tiled(const TiledArgs a) {
     __shared__ double4 s_ph[kCT][32];
     __shared__ double s_uvw[32][3];
@@ -75,7 +83,7 @@ __global__ __launch_bounds__(256) void k_direct_tiled(const TiledArgs a) {
     const double bu = s_uvw[sp][0] - s_uvw[sq][0];
     const double bv = s_uvw[sp][1] - s_uvw[sq][1];
 
-    const int f0 = chunk * SM_NCH;
+    const int f0 = chunk * NCH;
     const double *__restrict__ freqs = a.freqs;
     const double F0 = freqs[f0];
     const double DF = freqs[f0 + 1] - F0;
@@ -85,9 +93,9 @@ __global__ __launch_bounds__(256) void k_direct_tiled(const TiledArgs a) {
     const double4 *__restrict__ lmn4 = a.lmn;
     const double4 *__restrict__ shape = a.shape;
 
-    double acc[SM_NCH][8];
+    double acc[NCH][8];
 #pragma unroll
-    for (int k = 0; k < SM_NCH; ++k)
+    for (int k = 0; k < NCH; ++k)
 #pragma unroll
         for (int e = 0; e < 8; ++e) acc[k][e] = 0.0;
 
@@ -124,7 +132,7 @@ __global__ __launch_bounds__(256) void k_direct_tiled(const TiledArgs a) {
             }
             const double *cc8 = coh + ((size_t)c * a.nchan_pad + f0) * 8;
 #pragma unroll
-            for (int k = 0; k < SM_NCH; ++k) {
+            for (int k = 0; k < NCH; ++k) {
                 double pr_ = zr, pi_ = zi;
                 if (KIND == KIND_GAUSSIAN) {
                     pr_ *= env;
@@ -157,7 +165,7 @@ __global__ __launch_bounds__(256) void k_direct_tiled(const TiledArgs a) {
     const size_t nt = a.tile_map != nullptr ? (size_t)a.ntiles_bl : (size_t)a.ntiles;
     const size_t bl = (size_t)p * nt - (size_t)p * (p + 1) / 2 + (size_t)(q - p - 1);
 #pragma unroll
-    for (int k = 0; k < SM_NCH; ++k) {
+    for (int k = 0; k < NCH; ++k) {
         const int f = f0 + k;
         if (f >= a.nchan) break;
         const size_t o = (size_t)f * a.nbl + bl;
@@ -184,13 +192,16 @@ std::vector<int4> tile_pairs(int ntiles) {
     return out;
 }
 
-hipError_t launch_direct_tiled(Kind kind, const TiledArgs &a, hipStream_t stream) {
-    if (a.ncomp <= 0 || a.nbl <= 0 || a.nchan <= 0) return hipSuccess;
+hipError_t launch_direct_tiled(Kind kind, const TiledArgs &a_in, hipStream_t stream) {
+    if (a_in.ncomp <= 0 || a_in.nbl <= 0 || a_in.nchan <= 0) return hipSuccess;
+    TiledArgs a = a_in;
+    const int nch = kind == KIND_GAUSSIAN ? kTiledNchGaussian : kTiledNchPoint;
+    a.nchunks = (a.nchan + nch - 1) / nch;
     const int total = a.npairs * a.nchunks;
     if (kind == KIND_GAUSSIAN)
-        hipLaunchKernelGGL(k_direct_tiled<KIND_GAUSSIAN>, dim3(total), dim3(256), 0, stream, a);
+        hipLaunchKernelGGL((k_direct_tiled<KIND_GAUSSIAN, kTiledNchGaussian>), dim3(total), dim3(256), 0, stream, a);
     else
-        hipLaunchKernelGGL(k_direct_tiled<KIND_POINT>, dim3(total), dim3(256), 0, stream, a);
+        hipLaunchKernelGGL((k_direct_tiled<KIND_POINT, kTiledNchPoint>), dim3(total), dim3(256), 0, stream, a);
     return hipGetLastError();
 }
 

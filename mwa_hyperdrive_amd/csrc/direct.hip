@@ -3,13 +3,13 @@
 // V_pq(f) += sum_c (J_p S_c(f) J_q^H) e^{i (u L + v M + w N) f/c} [x envelope]
 // (src/model/cpu.rs:337-345,472-487,702-717; src/gpu/model.cu:99-366).
 //
-// Mapping: one lane per baseline, SM_NCH consecutive channels per lane, the component
+// Mapping: one lane per baseline, NCH consecutive channels per lane (kDirectNch), the component
 // loop serial per lane. Everything that depends only on (component, channel) is
 // wave-uniform: the coherency J S J^H (hoisted out of the baseline loop when every tile
 // shares one beam), LMN, envelope parameters and shapelet coefficients arrive through
 // scalar loads and SGPR operands, so the vector ALUs do only per-baseline work. Across a
 // chunk of evenly spaced channels the phase e^{i theta f_k / c} advances by a fixed
-// rotation, so two FP64 sincos per (baseline, component, chunk) replace SM_NCH of them;
+// rotation, so two FP64 sincos per (baseline, component, chunk) replace NCH of them;
 // the Gaussian envelope exp(q f_k^2) advances by a geometric recurrence in the same way.
 // Blocks are remapped so that all baseline blocks of a channel chunk land on the same XCD
 // and share that XCD's L2 copy of the chunk's coherency rows.
@@ -211,14 +211,14 @@ __device__ __forceinline__ void logical_block(int gx, int total, int &bx, int &c
     bx = b - chunk * gx;
 }
 
-template <int KIND, bool PER_TILE, bool EVEN, bool SPLIT>
+template <int KIND, bool PER_TILE, bool EVEN, bool SPLIT, int NCH>
 __global__ __launch_bounds__(256) void k_direct(const DirectArgs a, int gx, int total) {
     int bx, chunk;
     logical_block(gx, total, bx, chunk);
     // Component range: all, or one of csplit ranges (subset mode).
     int c_lo = 0, c_hi = a.ncomp;
     if (SPLIT) {
-        const int gy = (a.nchan + SM_NCH - 1) / SM_NCH;
+        const int gy = (a.nchan + NCH - 1) / NCH;
         const int split = chunk / gy;
         chunk -= split * gy;
         const int per = (a.ncomp + a.csplit - 1) / a.csplit;
@@ -230,7 +230,7 @@ __global__ __launch_bounds__(256) void k_direct(const DirectArgs a, int gx, int 
     const int bl_raw = bx * 256 + (int)threadIdx.x;
     const int li = bl_raw < nlanes ? bl_raw : nlanes - 1;
     const int bl = a.bl_subset != nullptr ? a.bl_subset[li] : li;
-    const int f0 = chunk * SM_NCH;
+    const int f0 = chunk * NCH;
 
     const double *__restrict__ coh = a.coh;
     const double4 *__restrict__ lmn4 = a.lmn;
@@ -246,9 +246,9 @@ __global__ __launch_bounds__(256) void k_direct(const DirectArgs a, int gx, int 
     int2 rows = make_int2(0, 0);
     if (PER_TILE) rows = a.beam.bl_rows[bl];
 
-    double acc[SM_NCH][8];
+    double acc[NCH][8];
 #pragma unroll
-    for (int k = 0; k < SM_NCH; ++k)
+    for (int k = 0; k < NCH; ++k)
 #pragma unroll
         for (int e = 0; e < 8; ++e) acc[k][e] = 0.0;
 
@@ -280,7 +280,7 @@ __global__ __launch_bounds__(256) void k_direct(const DirectArgs a, int gx, int 
         }
         const double *cc = coh + ((size_t)c * a.nchan_pad + f0) * 8;
 #pragma unroll
-        for (int k = 0; k < SM_NCH; ++k) {
+        for (int k = 0; k < NCH; ++k) {
             double pr, pi;
             if (EVEN) {
                 pr = zr;
@@ -335,7 +335,7 @@ __global__ __launch_bounds__(256) void k_direct(const DirectArgs a, int gx, int 
     if (bl_raw >= nlanes) return;
     if (SPLIT) {
 #pragma unroll
-        for (int k = 0; k < SM_NCH; ++k) {
+        for (int k = 0; k < NCH; ++k) {
             const int f = f0 + k;
             if (f >= a.nchan) break;
             double *dst = a.accum + ((size_t)f * a.nsub + li) * 8;
@@ -345,7 +345,7 @@ __global__ __launch_bounds__(256) void k_direct(const DirectArgs a, int gx, int 
         return;
     }
 #pragma unroll
-    for (int k = 0; k < SM_NCH; ++k) {
+    for (int k = 0; k < NCH; ++k) {
         const int f = f0 + k;
         if (f >= a.nchan) break;
         const size_t o = (size_t)f * a.nbl + bl;
@@ -360,20 +360,29 @@ __global__ __launch_bounds__(256) void k_direct(const DirectArgs a, int gx, int 
     }
 }
 
+// Channels per lane of the baseline kernel. Four channels hold half the accumulators of
+// SM_NCH = 8, one more wave per SIMD for points and Gaussians. Measured on the 256-tile,
+// 384-channel per-tile-beam case: points 929 -> 853 ms, Gaussians 336 -> 300 ms per timestep.
+// Shapelets take this kernel only for uneven channels or the drop-in path and keep SM_NCH.
+template <int KIND, bool PT>
+constexpr int kDirectNch = KIND != KIND_SHAPELET ? 4 : SM_NCH;
+static_assert(SM_NCH % 4 == 0, "chunks of 4 channels tile the chunks of SM_NCH that the padding covers");
+
 template <int KIND, bool PT>
 static hipError_t launch_direct_kind(const DirectArgs &a, bool even, hipStream_t stream) {
+    constexpr int NCH = kDirectNch<KIND, PT>;
     const int gx = ((a.bl_subset != nullptr ? a.nsub : a.nbl) + 255) / 256;
-    const int gy = (a.nchan + SM_NCH - 1) / SM_NCH;
+    const int gy = (a.nchan + NCH - 1) / NCH;
     const int total = gx * gy * a.csplit;
     // The split form is a separate instantiation so that the common kernels compile as before.
     if (a.csplit > 1 && even)
-        hipLaunchKernelGGL((k_direct<KIND, PT, true, true>), dim3(total), dim3(256), 0, stream, a, gx, total);
+        hipLaunchKernelGGL((k_direct<KIND, PT, true, true, NCH>), dim3(total), dim3(256), 0, stream, a, gx, total);
     else if (a.csplit > 1)
-        hipLaunchKernelGGL((k_direct<KIND, PT, false, true>), dim3(total), dim3(256), 0, stream, a, gx, total);
+        hipLaunchKernelGGL((k_direct<KIND, PT, false, true, NCH>), dim3(total), dim3(256), 0, stream, a, gx, total);
     else if (even)
-        hipLaunchKernelGGL((k_direct<KIND, PT, true, false>), dim3(total), dim3(256), 0, stream, a, gx, total);
+        hipLaunchKernelGGL((k_direct<KIND, PT, true, false, NCH>), dim3(total), dim3(256), 0, stream, a, gx, total);
     else
-        hipLaunchKernelGGL((k_direct<KIND, PT, false, false>), dim3(total), dim3(256), 0, stream, a, gx, total);
+        hipLaunchKernelGGL((k_direct<KIND, PT, false, false, NCH>), dim3(total), dim3(256), 0, stream, a, gx, total);
     return hipGetLastError();
 }
 

@@ -181,17 +181,27 @@ __global__ __launch_bounds__(256) void k_factored_coh32_classes(const FactoredAr
     for (int k = 0; k < 8; ++k) dst[k] = (float)(o[k] * sc);
 }
 
-// max |J_r S J_0^H| over classes, components and channels, for the scale.
+// max |J_r S J_0^H| over classes, components and channels, for the scale. A thread takes one
+// (channel, component) with components fastest, so the Jones reads of a wave are contiguous,
+// and loops over the classes with S and J_0 loaded once.
 __global__ void k_absmax_classes(const FactoredArgs a, unsigned long long *out) {
     double m = 0.0;
-    const size_t n = (size_t)a.nclass * a.ncomp * a.nchan;
+    const size_t n = (size_t)a.ncomp * a.nchan;
+    const BeamDesc &b = a.beam;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const int f = (int)(i % a.nchan);
-        const size_t rc = i / a.nchan;
-        double o[8];
-        class_coherency(a, (int)(rc / a.ncomp), (int)(rc % a.ncomp), f, o);
+        const int c = (int)(i % a.ncomp);
+        const int f = (int)(i / a.ncomp);
+        double sv[8], j0[8];
+        load_jones(reinterpret_cast<const JonesF64 *>(a.coh + ((size_t)c * a.nchan_pad + f) * 8), sv);
+        const size_t col = (size_t)b.freq_map[f] * b.n_dir + b.dir_offset + c;
+        load_jones(&b.jones[(size_t)a.class_row[0] * b.n_ufreq * b.n_dir + col], j0);
+        for (int r = 0; r < a.nclass; ++r) {
+            double jr[8], o[8];
+            load_jones(&b.jones[(size_t)a.class_row[r] * b.n_ufreq * b.n_dir + col], jr);
+            sandwich(jr, sv, j0, o);
 #pragma unroll
-        for (int k = 0; k < 8; ++k) m = fmax(m, fabs(o[k]));
+            for (int k = 0; k < 8; ++k) m = fmax(m, fabs(o[k]));
+        }
     }
     for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
     if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));

@@ -11,7 +11,8 @@
 //                                no beam), otherwise the instrumental flux S alone
 //   bl_uv [n][nbl]               double2 shapelet (u, v) metres, component as phase centre
 //   vis   [nchan][nbl]           JonesF32 (+=) and optionally JonesF64 (+=)
-// Channels are processed in chunks of SM_NCH; nchan_pad rounds nchan up to a whole chunk.
+// Channels are processed in chunks of at most SM_NCH (kernels pick 4 or 8); nchan_pad rounds
+// nchan up to a whole chunk of SM_NCH and adds one more, so every chunk's reads stay inside.
 #pragma once
 
 #include <hip/hip_runtime.h>

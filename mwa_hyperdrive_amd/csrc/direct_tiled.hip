@@ -251,13 +251,26 @@ __device__ __forceinline__ void shapelet_chan(const ShapeletChanArgs &a) {
         fmax_w = fmax(fmax_w, __shfl_xor(fmax_w, o));
     }
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // The wave's baseline is uniform; the next component's parameters are loaded one
+    // iteration ahead.
+    const int blu = __builtin_amdgcn_readfirstlane(bl);
+    double4 l_next = a.lmn[0], sh_next = a.shape[0];
+    double2 suv_next = a.bl_uv[blu];
+    int4 pr_next = a.prange[0];
     for (int c = 0; c < a.ncomp; ++c) {
-        const double4 l = a.lmn[c];
+        const double4 l = l_next, sh = sh_next;
+        const double2 suv = suv_next;
+        const int4 pr4 = pr_next;
+        {
+            const int cn = c + 1 < a.ncomp ? c + 1 : c;
+            l_next = a.lmn[cn];
+            sh_next = a.shape[cn];
+            suv_next = a.bl_uv[(size_t)cn * a.nbl + blu];
+            pr_next = a.prange[cn];
+        }
         const double theta = uvw.u * l.x + uvw.v * l.y + uvw.w * l.z;
         double zr, zi;
         fast_sincos(theta * fc, zi, zr);
-        const double4 sh = a.shape[c];
-        const double2 suv = a.bl_uv[(size_t)c * a.nbl + bl];
         const double xk = (suv.x * sh.x + suv.y * sh.y) * sh.z;
         const double yk = (suv.x * sh.y - suv.y * sh.x) * sh.w;
         const double x_pos = xk * freq + SM_SBF_C;
@@ -267,7 +280,6 @@ __device__ __forceinline__ void shapelet_chan(const ShapeletChanArgs &a) {
         const int iy = (int)fmin(fmax(fy, 0.0), (double)(SM_SBF_L - 2));
         const double wx = x_pos - fx, wy = y_pos - fy;
         double er, ei;
-        const int4 pr4 = a.prange[c];
         // The cells the wave touches: positions are monotone in frequency, so the extreme
         // channels bound them (the same expressions, so the same roundings).
         const double xa = xk * fmin_w + SM_SBF_C, xb = xk * fmax_w + SM_SBF_C;
@@ -278,6 +290,8 @@ __device__ __forceinline__ void shapelet_chan(const ShapeletChanArgs &a) {
         const int iy1 = (int)fmin(fmax(floor(fmax(ya, yb)), 0.0), (double)(SM_SBF_L - 2));
         const int xw = __builtin_amdgcn_readfirstlane(ix1 - ix0 + 2);
         const int yw = __builtin_amdgcn_readfirstlane(iy1 - iy0 + 2);
+        double cpre[8];
+        const double *s_c = a.coh + ((size_t)c * a.nchan_pad + f) * 8;
         if (xw <= kWin && yw <= kWin && pr4.w <= kRows) {
             // Stage rows 0 .. pr4.w - 1 of both windows in this wave's LDS, then interpolate
             // from LDS: one 16-byte read per axis per coefficient instead of two gathers.
@@ -316,6 +330,10 @@ __device__ __forceinline__ void shapelet_chan(const ShapeletChanArgs &a) {
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             const int jx = ix - bx0, jy = iy - by0;
+            if (!PER_TILE) {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) cpre[e] = s_c[e];
+            }
             double part[2];
 #pragma unroll
             for (int pt = 0; pt < 2; ++pt) {
@@ -364,6 +382,10 @@ __device__ __forceinline__ void shapelet_chan(const ShapeletChanArgs &a) {
             }
             er = part[0];
             ei = part[1];
+            if (!PER_TILE) {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) cpre[e] = s_c[e];
+            }
         }
         const double pr = zr * er - zi * ei;
         const double pi = zr * ei + zi * er;
@@ -380,7 +402,7 @@ __device__ __forceinline__ void shapelet_chan(const ShapeletChanArgs &a) {
             sandwich_t(j1, sv, j2, cj);
         } else {
 #pragma unroll
-            for (int e = 0; e < 8; ++e) cj[e] = s[e];
+            for (int e = 0; e < 8; ++e) cj[e] = cpre[e];
         }
 #pragma unroll
         for (int e = 0; e < 4; ++e) {

@@ -183,11 +183,42 @@ hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf
         t.ntiles_bl = ctx.ntiles;
         if (ev0 != nullptr) (void)hipEventRecord(ev0, stream);
         e = launch_direct_tiled(kind, t, stream);
-        // Pairs of a class-r tile with a class-0 tile: one shared coherency per segment, so
-        // the shared-beam form of the baseline kernel (no per-visibility Jones products).
-        thread_local DevBuf cseg;
+        // Pairs of a class-r tile with a class-0 tile. With a table of J_r S J_0^H for every
+        // class (within a memory cap): one launch of the row-by-column kernel. Otherwise one
+        // shared coherency per segment and the shared-beam form of the baseline kernel.
+        thread_local DevBuf cseg, ccls;
         const int nseg = 2 * (ctx.nclass - 1);
-        for (int sgi = 0; sgi < nseg && e == hipSuccess; ++sgi) {
+        const size_t ccls_bytes = (size_t)ctx.nclass * n * ctx.nchan_pad * 8 * sizeof(double);
+        const bool rect = ctx.n_other > 0 && ctx.d_other_tiles != nullptr && ctx.nclass > 1 &&
+                          ccls_bytes <= (size_t(24) << 30);
+        if (rect && e == hipSuccess) {
+            e = ccls.reserve(ccls_bytes);
+            if (e == hipSuccess)
+                e = launch_class_coherency(ctx.nclass, n, ctx.nchan, ctx.nchan_pad, coh.as<double>(), ctx.beam,
+                                           ctx.d_class_row, ccls.as<double>(), stream);
+            if (e == hipSuccess) {
+                RectArgs r;
+                r.n_row = ctx.n_other;
+                r.n_col = ctx.g0_n;
+                r.nbl = ctx.nbl;
+                r.ntiles_bl = ctx.ntiles;
+                r.nchan = ctx.nchan;
+                r.nchan_pad = ctx.nchan_pad;
+                r.ncomp = n;
+                r.row_tiles = ctx.d_other_tiles;
+                r.row_cls = ctx.d_other_cls;
+                r.col_tiles = ctx.d_g0_tiles;
+                r.tile_uvw = ctx.d_tile_uvw;
+                r.freqs = ctx.d_freqs_pad;
+                r.lmn = in.lmn;
+                r.shape = in.shape;
+                r.coh_cls = ccls.as<double>();
+                r.vis32 = vis32;
+                r.vis64 = vis64;
+                e = launch_direct_rect(kind, r, stream);
+            }
+        }
+        for (int sgi = 0; sgi < nseg && e == hipSuccess && !rect; ++sgi) {
             const int off = ctx.h_seg_off[sgi], cnt = ctx.h_seg_off[sgi + 1] - off;
             if (cnt <= 0) continue;
             const int r = 1 + sgi / 2;

@@ -205,6 +205,173 @@ hipError_t launch_direct_tiled(Kind kind, const TiledArgs &a_in, hipStream_t str
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ row tiles x class 0
+// k_direct_rect: a block owns 4 row tiles (one per wave) and 64 class-0 column tiles (one
+// per lane) for one channel chunk. Every lane of a wave shares its row's class, so the
+// coherency J_r S J_0^H arrives through scalar loads as in k_direct_tiled. The kernel forms
+// V_pq for the ordered pair (p = row tile, q = column tile); where the array orders the
+// baseline the other way (q < p) it stores V_qp = V_pq^H, which holds because the
+// instrumental coherency S is Hermitian (XY and YX are conjugates).
+constexpr int kRC = 16;  // components per LDS stage
+constexpr int kRT = 4 + 64;  // tile slots per block: 4 rows, then 64 columns
+
+template <int KIND, int NCH>
+__global__ __launch_bounds__(256) void k_direct_rect(const RectArgs a) {
+    __shared__ double4 s_ph[kRC][kRT];
+    __shared__ double s_uvw[kRT][3];
+    const int nrg = (a.n_row + 3) / 4, ncg = (a.n_col + 63) / 64;
+    const int b = blockIdx.x;
+    const int chunk = b / (nrg * ncg);
+    const int rem = b - chunk * (nrg * ncg);
+    const int rg = rem / ncg, cg = rem - rg * ncg;
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    if (tid < kRT) {
+        int t;
+        if (tid < 4) {
+            const int i = rg * 4 + tid;
+            t = a.row_tiles[i < a.n_row ? i : a.n_row - 1];
+        } else {
+            const int j = cg * 64 + tid - 4;
+            t = a.col_tiles[j < a.n_col ? j : a.n_col - 1];
+        }
+        s_uvw[tid][0] = a.tile_uvw[3 * t];
+        s_uvw[tid][1] = a.tile_uvw[3 * t + 1];
+        s_uvw[tid][2] = a.tile_uvw[3 * t + 2];
+    }
+    const int ri = rg * 4 + w, cj = cg * 64 + lane;
+    const bool row_ok = ri < a.n_row;
+    const bool valid = row_ok && cj < a.n_col;
+    const int cls = __builtin_amdgcn_readfirstlane(row_ok ? a.row_cls[ri] : 0);
+    __syncthreads();
+    const int sp = w, sq = 4 + lane;
+    const double bu = s_uvw[sp][0] - s_uvw[sq][0];
+    const double bv = s_uvw[sp][1] - s_uvw[sq][1];
+
+    const int f0 = chunk * NCH;
+    const double *__restrict__ freqs = a.freqs;
+    const double F0 = freqs[f0];
+    const double DF = freqs[f0 + 1] - F0;
+    const double f0c = F0 / SM_VEL_C;
+    const double dfc = DF / SM_VEL_C;
+    const double *__restrict__ coh = a.coh_cls + (size_t)cls * a.ncomp * a.nchan_pad * 8;
+    const double4 *__restrict__ lmn4 = a.lmn;
+    const double4 *__restrict__ shape = a.shape;
+
+    double acc[NCH][8];
+#pragma unroll
+    for (int k = 0; k < NCH; ++k)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) acc[k][e] = 0.0;
+
+    for (int c0 = 0; c0 < a.ncomp; c0 += kRC) {
+        const int nc = a.ncomp - c0 < kRC ? a.ncomp - c0 : kRC;
+        for (int e = tid; e < kRC * kRT; e += 256) {
+            const int cc = e / kRT, slot = e - cc * kRT;
+            double4 ph = make_double4(1.0, 0.0, 1.0, 0.0);
+            if (cc < nc) {
+                const double4 l = lmn4[c0 + cc];
+                const double th = s_uvw[slot][0] * l.x + s_uvw[slot][1] * l.y + s_uvw[slot][2] * l.z;
+                fast_sincos(th * f0c, ph.y, ph.x);
+                fast_sincos(th * dfc, ph.w, ph.z);
+            }
+            s_ph[cc][slot] = ph;
+        }
+        __syncthreads();
+        if (row_ok) {
+            for (int cc = 0; cc < nc; ++cc) {
+                const int c = c0 + cc;
+                const double4 A = s_ph[cc][sp];
+                const double4 B = s_ph[cc][sq];
+                double zr = A.x * B.x + A.y * B.y;
+                double zi = A.y * B.x - A.x * B.y;
+                const double rr = A.z * B.z + A.w * B.w;
+                const double ri_ = A.w * B.z - A.z * B.w;
+                double env = 1.0, er = 1.0, eg = 1.0;
+                if (KIND == KIND_GAUSSIAN) {
+                    const double4 g = shape[c];
+                    const double qf = g.x * (bu * bu) + g.y * (bu * bv) + g.z * (bv * bv);
+                    env = exp(qf * (F0 * F0));
+                    er = exp(qf * (DF * (2.0 * F0 + DF)));
+                    eg = exp(qf * (2.0 * DF * DF));
+                }
+                const double *cc8 = coh + ((size_t)c * a.nchan_pad + f0) * 8;
+#pragma unroll
+                for (int k = 0; k < NCH; ++k) {
+                    double pr_ = zr, pi_ = zi;
+                    if (KIND == KIND_GAUSSIAN) {
+                        pr_ *= env;
+                        pi_ *= env;
+                    }
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const double cr = cc8[k * 8 + 2 * e], ci = cc8[k * 8 + 2 * e + 1];
+                        acc[k][2 * e] = fma(cr, pr_, acc[k][2 * e]);
+                        acc[k][2 * e] = fma(-ci, pi_, acc[k][2 * e]);
+                        acc[k][2 * e + 1] = fma(cr, pi_, acc[k][2 * e + 1]);
+                        acc[k][2 * e + 1] = fma(ci, pr_, acc[k][2 * e + 1]);
+                    }
+                    const double nr = zr * rr - zi * ri_;
+                    zi = zr * ri_ + zi * rr;
+                    zr = nr;
+                    if (KIND == KIND_GAUSSIAN) {
+                        env *= er;
+                        er *= eg;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!valid) return;
+    const int p = a.row_tiles[ri], q = a.col_tiles[cj];
+    const bool flip = q < p;  // array baseline (q, p): store V_pq^H
+    const int lo = flip ? q : p, hi = flip ? p : q;
+    const size_t nt = (size_t)a.ntiles_bl;
+    const size_t bl = (size_t)lo * nt - (size_t)lo * (lo + 1) / 2 + (size_t)(hi - lo - 1);
+#pragma unroll
+    for (int k = 0; k < NCH; ++k) {
+        const int f = f0 + k;
+        if (f >= a.nchan) break;
+        double v[8];
+        if (flip) {
+            // (V^H)_ij = conj(V_ji): XX, YX, XY, YY with the imaginary parts negated.
+            v[0] = acc[k][0];
+            v[1] = -acc[k][1];
+            v[2] = acc[k][4];
+            v[3] = -acc[k][5];
+            v[4] = acc[k][2];
+            v[5] = -acc[k][3];
+            v[6] = acc[k][6];
+            v[7] = -acc[k][7];
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) v[e] = acc[k][e];
+        }
+        const size_t o = (size_t)f * a.nbl + bl;
+        float *v32 = reinterpret_cast<float *>(&a.vis32[o]);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v32[e] += (float)v[e];
+        if (a.vis64 != nullptr) {
+            double *v64 = reinterpret_cast<double *>(&a.vis64[o]);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) v64[e] += v[e];
+        }
+    }
+}
+
+hipError_t launch_direct_rect(Kind kind, const RectArgs &a, hipStream_t stream) {
+    if (a.ncomp <= 0 || a.n_row <= 0 || a.n_col <= 0 || a.nchan <= 0) return hipSuccess;
+    const int nch = kind == KIND_GAUSSIAN ? kTiledNchGaussian : kTiledNchPoint;
+    const long total = (long)((a.n_row + 3) / 4) * ((a.n_col + 63) / 64) * ((a.nchan + nch - 1) / nch);
+    if (kind == KIND_GAUSSIAN)
+        hipLaunchKernelGGL((k_direct_rect<KIND_GAUSSIAN, kTiledNchGaussian>), dim3((unsigned)total), dim3(256), 0,
+                           stream, a);
+    else
+        hipLaunchKernelGGL((k_direct_rect<KIND_POINT, kTiledNchPoint>), dim3((unsigned)total), dim3(256), 0, stream,
+                           a);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ shapelets
 template <bool PER_TILE>
 __device__ __forceinline__ void shapelet_chan(const ShapeletChanArgs &a) {

@@ -227,6 +227,8 @@ struct SmModeller {
     // Class-0 tiles for the tile-pair kernel: their tile-group pairs, gathered UVWs and the
     // baselines with a tile outside class 0.
     int g0_n0 = 0, g0_npairs = 0;
+    DevBuf d_other_tiles, d_other_cls;
+    int n_other = 0;
     DevBuf d_g0_pairs, d_g0_only_uvw, d_notg0_bl;
     std::vector<int> h_seg_off, h_class_row;  // segments of d_notg0_bl; beam row per class
     int npairs = 0;
@@ -658,6 +660,18 @@ extern "C" const char *sm_new(const SmObsParams *p, const SmComponent *comps, co
                 SM_NEW_TRY(upload(m->d_g0_map, map));
                 SM_NEW_TRY(upload(m->d_g0_rest, rest));
                 SM_NEW_TRY(upload(m->d_g0_rclass, rclass));
+                {
+                    std::vector<int> ot, oc;
+                    for (int t = g0b * 64; t < nft; ++t) {
+                        ot.push_back(ftile[t]);
+                        oc.push_back(rclass[t]);
+                    }
+                    m->n_other = (int)ot.size();
+                    if (m->n_other > 0) {
+                        SM_NEW_TRY(upload(m->d_other_tiles, ot));
+                        SM_NEW_TRY(upload(m->d_other_cls, oc));
+                    }
+                }
                 SM_NEW_TRY(upload(m->d_g0_class_row, class_row));
                 SM_NEW_TRY(upload(m->d_class_of_row, class_of_row));
             }
@@ -757,6 +771,11 @@ static const char *model_timestep_device(SmModeller *m, double lst, double latit
             ctx.d_g0_uvw = m->d_g0_only_uvw.as<double>();
             ctx.d_g0_pairs = m->d_g0_pairs.as<int4>();
             ctx.d_notg0_bl = m->d_notg0_bl.as<int>();
+            if (m->n_other > 0) {
+                ctx.d_other_tiles = m->d_other_tiles.as<int>();
+                ctx.d_other_cls = m->d_other_cls.as<int>();
+                ctx.n_other = m->n_other;
+            }
         }
     }
     const Kind kinds[3] = {KIND_POINT, KIND_GAUSSIAN, KIND_SHAPELET};

@@ -45,6 +45,11 @@ struct ModelCtx {
     const double *d_g0_uvw = nullptr;   // [g0_n][3] their tile UVWs
     const int4 *d_g0_pairs = nullptr;   // tile-group pairs over the class-0 tiles
     const int *d_notg0_bl = nullptr;
+    // Array tiles outside class 0 and their classes (pairs with a class-0 tile take the
+    // row-by-column kernel).
+    const int *d_other_tiles = nullptr;
+    const int *d_other_cls = nullptr;
+    int n_other = 0;
     const int *h_seg_off = nullptr;
     const int *h_class_row = nullptr;
     const int *d_class_row = nullptr;   // [nclass] beam row of each class

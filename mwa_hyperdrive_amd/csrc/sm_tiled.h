@@ -40,7 +40,25 @@ struct ShapeletChanArgs {
     JonesF64 *vis64;
 };
 
+// Pairs of a tile outside beam class 0 (rows) with a class-0 tile (columns): one wave per
+// row tile and 64 column tiles, so the row's class, and with it the coherency
+// J_r S J_0^H, is wave-uniform.
+struct RectArgs {
+    int n_row, n_col, nbl, ntiles_bl, nchan, nchan_pad, ncomp;
+    const int *row_tiles;     // [n_row] array tiles outside class 0
+    const int *row_cls;       // [n_row] their classes (1 .. nclass - 1)
+    const int *col_tiles;     // [n_col] class-0 array tiles
+    const double *tile_uvw;   // [ntiles_bl][3] metres
+    const double *freqs;      // [nchan_pad]
+    const double4 *lmn;
+    const double4 *shape;
+    const double *coh_cls;    // [nclass][ncomp][nchan_pad][8], J_r S J_0^H
+    JonesF32 *vis32;
+    JonesF64 *vis64;
+};
+
 std::vector<int4> tile_pairs(int ntiles);
+hipError_t launch_direct_rect(Kind kind, const RectArgs &a, hipStream_t stream);
 hipError_t launch_direct_tiled(Kind kind, const TiledArgs &a, hipStream_t stream);
 hipError_t launch_shapelet_chan(bool per_tile, const ShapeletChanArgs &a, hipStream_t stream);
 hipError_t launch_tile_uvw_from_baselines(int ntiles, const UVW *d_uvw, double *d_out, hipStream_t stream);

@@ -71,26 +71,33 @@ __global__ void k_coherency(int ncomp, int nchan, int nchan_pad, const double *_
 
 __global__ void k_class_coherency(int nclass, int ncomp, int nchan, int nchan_pad, const double *__restrict__ s_tab,
                                   BeamDesc beam, const int *__restrict__ class_row, double *__restrict__ out) {
+    // One thread per (component, channel), looping over the classes: S and J_0 are read once
+    // per (component, channel) instead of once per class.
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (size_t)nclass * ncomp * nchan_pad) return;
+    if (idx >= (size_t)ncomp * nchan_pad) return;
     const int f = (int)(idx % nchan_pad);
-    const size_t rc = idx / nchan_pad;
-    const int c = (int)(rc % ncomp);
-    const int r = (int)(rc / ncomp);
-    double *o = out + idx * 8;
+    const int c = (int)(idx / nchan_pad);
+    const size_t cstride = (size_t)ncomp * nchan_pad * 8;
     if (f >= nchan) {
+        for (int r = 0; r < nclass; ++r) {
+            double *o = out + r * cstride + idx * 8;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) o[e] = 0.0;
+            for (int e = 0; e < 8; ++e) o[e] = 0.0;
+        }
         return;
     }
-    double sv[8], ja[8], jb[8], v[8];
+    double sv[8], jb[8];
     load_jones(reinterpret_cast<const JonesF64 *>(s_tab + ((size_t)c * nchan_pad + f) * 8), sv);
     const size_t col = (size_t)beam.freq_map[f] * beam.n_dir + beam.dir_offset + c;
-    load_jones(&beam.jones[(size_t)class_row[r] * beam.n_ufreq * beam.n_dir + col], ja);
     load_jones(&beam.jones[(size_t)class_row[0] * beam.n_ufreq * beam.n_dir + col], jb);
-    sandwich(ja, sv, jb, v);
+    for (int r = 0; r < nclass; ++r) {
+        double ja[8], v[8];
+        load_jones(&beam.jones[(size_t)class_row[r] * beam.n_ufreq * beam.n_dir + col], ja);
+        sandwich(ja, sv, jb, v);
+        double *o = out + r * cstride + idx * 8;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) o[e] = v[e];
+        for (int e = 0; e < 8; ++e) o[e] = v[e];
+    }
 }
 
 __global__ void k_pair_coherency(int ncomp, int nchan, int nchan_pad, const double *__restrict__ s_tab, BeamDesc beam,
@@ -127,8 +134,8 @@ hipError_t launch_pair_coherency(int ncomp, int nchan, int nchan_pad, const doub
 hipError_t launch_class_coherency(int nclass, int ncomp, int nchan, int nchan_pad, const double *d_s,
                                   const BeamDesc &beam, const int *d_class_row, double *d_coh_cls,
                                   hipStream_t stream) {
-    const size_t n = (size_t)nclass * ncomp * nchan_pad;
-    if (n == 0) return hipSuccess;
+    const size_t n = (size_t)ncomp * nchan_pad;
+    if (n == 0 || nclass <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_class_coherency, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, nclass, ncomp,
                        nchan, nchan_pad, d_s, beam, d_class_row, d_coh_cls);
     return hipGetLastError();

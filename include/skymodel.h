@@ -27,22 +27,22 @@ extern "C" {
 
 /* ------------------------------------------------------------------ 1. kernel drop-ins */
 
-/* Replaces `model_points` (src/gpu/model.h:324-325; launched at src/model/gpu.rs:913-945).
+/* Replaces `model_points` (src/gpu/model.h:33-34; launched at src/model/gpu.rs:913-945).
  * d_beam_jones is [num_unique_tiles][num_unique_beam_freqs][directions] with directions
  * ordered power law, curved power law, list (src/model/gpu.rs:889-910). */
 const char *model_points(const Points *comps, const Addresses *a, const UVW *d_uvws, const JonesF64 *d_beam_jones,
                          JonesF32 *d_vis_fb);
 
-/* Replaces `model_gaussians` (src/gpu/model.h:332-333). */
+/* Replaces `model_gaussians` (src/gpu/model.h:41-42). */
 const char *model_gaussians(const Gaussians *comps, const Addresses *a, const UVW *d_uvws,
                             const JonesF64 *d_beam_jones, JonesF32 *d_vis_fb);
 
-/* Replaces `model_shapelets` (src/gpu/model.h:340-341). The shapelet UVs inside `comps`
+/* Replaces `model_shapelets` (src/gpu/model.h:49-50). The shapelet UVs inside `comps`
  * are [baseline][component] metres, as the reference's host computes them. */
 const char *model_shapelets(const Shapelets *comps, const Addresses *a, const UVW *d_uvws,
                             const JonesF64 *d_beam_jones, JonesF32 *d_vis_fb);
 
-/* Replaces `model_autos` (src/gpu/model.h:347). vis_fb is [freq][tile]. */
+/* Replaces `model_autos` (src/gpu/model.h:56). vis_fb is [freq][tile]. */
 const char *model_autos(int num_tiles, int num_freqs, const double *freqs, const Points *points,
                         const Gaussians *gaussians, const Shapelets *shapelets, const JonesF64 *beam_jones,
                         const int *tile_map, const int *freq_map, int num_fee_freqs,

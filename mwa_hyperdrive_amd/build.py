@@ -12,6 +12,7 @@ import os
 import shutil
 import subprocess
 import sys
+import tempfile
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
@@ -31,15 +32,22 @@ def hipcc() -> str:
 
 def build(verbose: bool = False) -> Path:
     srcs = [str(CSRC / s) for s in SOURCES if (CSRC / s).exists()]
+    # Objects go to a private temporary folder, so concurrent builds never share files
+    # and nothing but the library lands in the source tree.
+    with tempfile.TemporaryDirectory(prefix="skymodel_build_") as tmpdir:
+        return _build(srcs, Path(tmpdir), verbose)
+
+
+def _build(srcs, tmpdir: Path, verbose: bool) -> Path:
     objs = []
     jobs = []
     for s in srcs:
-        obj = str(Path(s).with_suffix(".o"))
+        obj = str(tmpdir / Path(s).with_suffix(".o").name)
         cmd = [hipcc(), f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-c",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
                "-Wno-unused-command-line-argument", f"-I{ROOT / 'include'}", "-o", obj, s]
         jobs.append((cmd, obj))
-    procs = [(subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT), obj, cmd) for cmd, obj in jobs]
+    procs = [(subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, cwd=tmpdir), obj, cmd) for cmd, obj in jobs]
     for p, obj, cmd in procs:
         out, _ = p.communicate()
         text = out.decode(errors="replace")
@@ -48,17 +56,15 @@ def build(verbose: bool = False) -> Path:
         if verbose and text.strip():
             print(text)
         objs.append(obj)
-    tmp = str(LIB) + ".tmp"
+    tmp = str(tmpdir / LIB.name)
     link = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
-    r = subprocess.run(link, capture_output=True, text=True)
+    r = subprocess.run(link, capture_output=True, text=True, cwd=tmpdir)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, LIB)
-    for o in objs:
-        try:
-            os.remove(o)
-        except OSError:
-            pass
+    # Copy next to the target, then rename: the library is replaced in one step.
+    staged = str(LIB) + f".{os.getpid()}.tmp"
+    shutil.copyfile(tmp, staged)
+    os.replace(staged, LIB)
     return LIB
 
 

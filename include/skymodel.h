@@ -169,6 +169,22 @@ const char *sm_synchronize(void);
  * kernels everywhere. Process-wide; used to compare the two. */
 const char *sm_set_direct_variant(int32_t variant);
 
+/* Limit in bytes on the per-beam-row-class coherency tables the per-tile-beam paths build
+ * (J_r S J_0^H for every class; 64 B per class, component and padded channel). Above it, or
+ * when the allocation fails, those paths form the coherency per baseline segment instead,
+ * with identical results. Negative (default): the device's free memory less 1 GiB. 0 always
+ * takes the per-segment path. Process-wide. No counterpart in the reference. */
+const char *sm_set_class_table_limit(int64_t bytes);
+
+/* Reads back the flux parameters the modeller holds on the device for one component of a
+ * bucket (type x flux kind, components in the modeller's order): power laws and curved power
+ * laws as the instrumental flux density at 150 MHz with the spectral index (and curvature q),
+ * the curved-power-law index re-referenced to 150 MHz; lists as the instrumental flux density
+ * estimated at channel `chan`. si and q are nullable. The reference's regression test of the
+ * re-reference reads the same device buffers (src/model/tests/gpu.rs:558-601). */
+const char *sm_get_flux_params(SmModeller *m, int32_t comp_type, int32_t flux_type, int32_t index, int32_t chan,
+                               JonesF64 *fd, double *si, double *q);
+
 /* ------------------------------------------------------------------ beam */
 /* Analytic MWA-like beam on the device: out[tile][freq][dir] JonesF64 for the given unique
  * tiles (delays [n_tiles][16], gains [n_tiles][32], host), frequencies (host, Hz) and

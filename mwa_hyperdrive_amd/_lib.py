@@ -142,6 +142,8 @@ SIGNATURES = {
     "sm_set_timing": [_P, _I],
     "sm_synchronize": [],
     "sm_set_direct_variant": [_I],
+    "sm_set_class_table_limit": [C.c_int64],
+    "sm_get_flux_params": [_P, _I, _I, _I, _I, _P, C.POINTER(_D), C.POINTER(_D)],
     "sm_analytic_beam": [_I, _P, _P, _I, _P, _I, _P, _P, _P],
     "sm_model_timestep_factored": [_P, _D, _D, _I, _P, _P],
     "sm_selftest_mfma_layout": [_P, _P, _P],

@@ -2,6 +2,7 @@
 // shared per-component-type driver used by both these and the SkyModeller handle.
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "sm_abi_common.h"
@@ -11,6 +12,7 @@ namespace sm {
 
 hipError_t DevBuf::reserve(size_t n) {
     if (n <= bytes && ptr != nullptr) return hipSuccess;
+    if (borrowed) return hipErrorOutOfMemory;
     release();
     size_t want = n < 256 ? 256 : n;
     hipError_t e = hipMalloc(&ptr, want);
@@ -24,9 +26,10 @@ hipError_t DevBuf::reserve(size_t n) {
 }
 
 void DevBuf::release() {
-    if (ptr != nullptr) (void)hipFree(ptr);
+    if (ptr != nullptr && !borrowed) (void)hipFree(ptr);
     ptr = nullptr;
     bytes = 0;
+    borrowed = false;
 }
 
 int pad_channels(int nchan) { return ((nchan + SM_NCH - 1) / SM_NCH) * SM_NCH + SM_NCH; }
@@ -79,11 +82,11 @@ double4 host_shapelet_shape(GaussianParams g) { return shapelet_shape(g); }
 
 // ------------------------------------------------------------------ shared driver
 // The baseline kernel on the subset a.bl_subset / a.nsub, splitting the component loop when
-// the subset fills too few lanes (one FP64 accumulation buffer per thread of the caller).
-static hipError_t launch_subset(Kind kind, bool per_tile, bool even, DirectArgs a, hipStream_t stream) {
+// the subset fills too few lanes (FP64 partial sums in accum).
+static hipError_t launch_subset(Kind kind, bool per_tile, bool even, DirectArgs a, DevBuf &accum,
+                                hipStream_t stream) {
     a.csplit = direct_subset_split(a.nsub, a.nchan, a.ncomp);
     if (a.csplit > 1) {
-        thread_local DevBuf accum;
         hipError_t e = accum.reserve(sizeof(double) * 8 * (size_t)a.nchan * a.nsub);
         if (e != hipSuccess) return e;
         a.accum = accum.as<double>();
@@ -91,8 +94,37 @@ static hipError_t launch_subset(Kind kind, bool per_tile, bool even, DirectArgs 
     return launch_direct(kind, per_tile, even, a, stream);
 }
 
-hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf &coh, JonesF32 *vis32,
-                    JonesF64 *vis64, hipStream_t stream, hipEvent_t ev0, hipEvent_t ev1) {
+long long g_class_table_limit = -1;
+
+// Reserves a per-class coherency table of `bytes` in buf if it is within the limit and the
+// allocation succeeds; false sends the caller to its per-segment path. A failed allocation
+// is cleared from the runtime's last-error state, so it does not surface later.
+static bool reserve_class_table(DevBuf &buf, size_t bytes) {
+    if (g_class_table_limit >= 0 && bytes > (size_t)g_class_table_limit) return false;
+    if (buf.ptr != nullptr && buf.bytes >= bytes) return true;
+    size_t limit;
+    if (g_class_table_limit >= 0) {
+        limit = (size_t)g_class_table_limit;
+    } else {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
+        // What buf already holds is released before the larger allocation.
+        free_b += buf.bytes;
+        limit = free_b > (size_t(1) << 30) ? free_b - (size_t(1) << 30) : 0;
+    }
+    if (bytes > limit) return false;
+    if (buf.reserve(bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return true;
+}
+
+hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf &coh, Scratch &scratch,
+                    JonesF32 *vis32, JonesF64 *vis64, hipStream_t stream, hipEvent_t ev0, hipEvent_t ev1) {
     const int n = in.fb.n_pl + in.fb.n_cpl + in.fb.n_list;
     if (n == 0 || ctx.nbl == 0 || ctx.nchan == 0) return hipSuccess;
     hipError_t e = coh.reserve((size_t)n * ctx.nchan_pad * 8 * sizeof(double));
@@ -102,25 +134,24 @@ hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf
                          stream);
     if (e != hipSuccess) return e;
     // Per-tile shapelets with beam-row classes: J_r S J_0^H per class for the channel-per-lane
-    // kernel, when the table fits a cap. (Measured on the 256-tile case: that kernel runs 28%
-    // faster. The baseline-per-lane kernels ran 2.2x slower with the same lookup, whether
-    // compiled into them or as a separate instantiation, so they keep forming J_p S J_q^H.)
+    // kernel, when the table fits (otherwise the kernel forms J_p S J_q^H per visibility).
+    // (Measured on the 256-tile case: that kernel runs 28% faster. The baseline-per-lane
+    // kernels ran 2.2x slower with the same lookup, whether compiled into them or as a
+    // separate instantiation, so they keep forming J_p S J_q^H.)
     BeamDesc beam = ctx.beam;
     const size_t cls_bytes = (size_t)ctx.nclass * n * ctx.nchan_pad * 8 * sizeof(double);
     if (kind == KIND_SHAPELET && g_direct_variant != 1 && ctx.d_bl_subset == nullptr && per_tile &&
-        ctx.nclass > 0 && ctx.d_class_row != nullptr && ctx.class_row0 >= 0 && cls_bytes <= (size_t(24) << 30)) {
-        thread_local DevBuf cls;
-        e = cls.reserve(cls_bytes);
-        if (e != hipSuccess) return e;
+        ctx.nclass > 0 && ctx.d_class_row != nullptr && ctx.class_row0 >= 0 &&
+        reserve_class_table(scratch.cls, cls_bytes)) {
         e = launch_class_coherency(ctx.nclass, n, ctx.nchan, ctx.nchan_pad, coh.as<double>(), ctx.beam, ctx.d_class_row,
-                                   cls.as<double>(), stream);
+                                   scratch.cls.as<double>(), stream);
         if (e != hipSuccess) return e;
-        beam.coh_cls = cls.as<double>();
+        beam.coh_cls = scratch.cls.as<double>();
         beam.class_of_row = ctx.d_class_of_row;
         beam.row0 = ctx.class_row0;
     }
-    // Shapelet coefficients in the kernels' packed form (per thread: handles are not shared).
-    thread_local DevBuf prange, packed;
+    // Shapelet coefficients in the kernels' packed form.
+    DevBuf &prange = scratch.prange, &packed = scratch.packed;
     const int4 *d_prange = nullptr;
     const PackedCoeff *d_packed = nullptr;
     if (kind == KIND_SHAPELET) {
@@ -156,7 +187,7 @@ hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf
     if (g0_tiled) {
         // Pairs of two class-0 tiles: J_0 S J_0^H, shared by all of them, on the tile-pair
         // kernel over the class-0 tiles; the other baselines: per-tile kernel on a subset.
-        thread_local DevBuf coh0;
+        DevBuf &coh0 = scratch.coh0;
         e = coh0.reserve((size_t)n * ctx.nchan_pad * 8 * sizeof(double));
         if (e != hipSuccess) return e;
         BeamDesc b0 = ctx.beam;
@@ -184,18 +215,17 @@ hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf
         if (ev0 != nullptr) (void)hipEventRecord(ev0, stream);
         e = launch_direct_tiled(kind, t, stream);
         // Pairs of a class-r tile with a class-0 tile. With a table of J_r S J_0^H for every
-        // class (within a memory cap): one launch of the row-by-column kernel. Otherwise one
-        // shared coherency per segment and the shared-beam form of the baseline kernel.
-        thread_local DevBuf cseg, ccls;
+        // class (when it fits, g_class_table_limit): one launch of the row-by-column kernel.
+        // Otherwise one shared coherency per segment and the shared-beam form of the
+        // baseline kernel.
+        DevBuf &cseg = scratch.cseg, &ccls = scratch.ccls;
         const int nseg = 2 * (ctx.nclass - 1);
         const size_t ccls_bytes = (size_t)ctx.nclass * n * ctx.nchan_pad * 8 * sizeof(double);
-        const bool rect = ctx.n_other > 0 && ctx.d_other_tiles != nullptr && ctx.nclass > 1 &&
-                          ccls_bytes <= (size_t(24) << 30);
-        if (rect && e == hipSuccess) {
-            e = ccls.reserve(ccls_bytes);
-            if (e == hipSuccess)
-                e = launch_class_coherency(ctx.nclass, n, ctx.nchan, ctx.nchan_pad, coh.as<double>(), ctx.beam,
-                                           ctx.d_class_row, ccls.as<double>(), stream);
+        const bool rect = e == hipSuccess && ctx.n_other > 0 && ctx.d_other_tiles != nullptr && ctx.nclass > 1 &&
+                          reserve_class_table(ccls, ccls_bytes);
+        if (rect) {
+            e = launch_class_coherency(ctx.nclass, n, ctx.nchan, ctx.nchan_pad, coh.as<double>(), ctx.beam,
+                                       ctx.d_class_row, ccls.as<double>(), stream);
             if (e == hipSuccess) {
                 RectArgs r;
                 r.n_row = ctx.n_other;
@@ -233,7 +263,7 @@ hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf
             u.coh = cseg.as<double>();
             u.bl_subset = ctx.d_notg0_bl + off;
             u.nsub = cnt;
-            e = launch_subset(kind, false, ctx.even, u, stream);
+            e = launch_subset(kind, false, ctx.even, u, scratch.accum, stream);
         }
         // Pairs of two tiles outside class 0: per-tile J_p S J_q^H.
         const int off = ctx.h_seg_off[nseg], cnt = ctx.h_seg_off[nseg + 1] - off;
@@ -241,14 +271,14 @@ hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf
             DirectArgs u = a;
             u.bl_subset = ctx.d_notg0_bl + off;
             u.nsub = cnt;
-            e = launch_subset(kind, true, ctx.even, u, stream);
+            e = launch_subset(kind, true, ctx.even, u, scratch.accum, stream);
         }
         if (ev1 != nullptr) (void)hipEventRecord(ev1, stream);
         return e != hipSuccess ? e : hipGetLastError();
     }
     if (ev0 != nullptr) (void)hipEventRecord(ev0, stream);
     if (ctx.d_bl_subset != nullptr) {
-        e = launch_subset(kind, per_tile, ctx.even, a, stream);
+        e = launch_subset(kind, per_tile, ctx.even, a, scratch.accum, stream);
     } else if (g_direct_variant == 0 && kind != KIND_SHAPELET && !per_tile && ctx.even && ctx.d_tile_uvw != nullptr &&
         ctx.d_pairs != nullptr && ctx.ntiles >= 2) {
         TiledArgs t;
@@ -300,20 +330,23 @@ int g_direct_variant = 0;
 // ------------------------------------------------------------------ ABI context
 namespace {
 
+// Work buffers of the kernel-level drop-ins. One set per calling thread, so that threads
+// calling the drop-ins on their own buffers never share scratch memory.
 struct AbiState {
     DevBuf freqs_pad, lmn, shape, coh, bl_rows, bl_uv, coeffs, coeff_range, sbf, accum, tile_rows, tile_uvw, pairs;
+    Scratch scratch;
     std::vector<double> h_freqs_pad;
     int pairs_ntiles = -1, npairs = 0;
 };
 AbiState &abi_state() {
-    static AbiState s;
+    thread_local AbiState s;
     return s;
 }
 
 // Builds the per-call context from an Addresses struct: frequencies (padded to whole
 // chunks), and whether every unflagged tile uses the same beam row.
-const char *abi_ctx(const Addresses *a, const UVW *d_uvws, const JonesF64 *d_beam, int n_dir, ModelCtx &ctx) {
-    AbiState &st = abi_state();
+const char *abi_ctx(AbiState &st, const Addresses *a, const UVW *d_uvws, const JonesF64 *d_beam, int n_dir,
+                    ModelCtx &ctx) {
     ctx = ModelCtx{};
     ctx.nbl = a->num_baselines;
     ctx.nchan = a->num_freqs;
@@ -413,15 +446,13 @@ const char *gather_shape(Kind kind, int n_pl, const GaussianParams *pl, int n_cp
 
 using namespace sm;
 
-extern "C" const char *sm_model_points_f64(const Points *p, const Addresses *a, const UVW *d_uvws,
-                                           const JonesF64 *d_beam_jones, JonesF32 *d_vis_fb, JonesF64 *d_vis_fb_f64) {
-    if (p == nullptr || a == nullptr) return "null argument";
+// The body of sm_model_points_f64 over the work buffers in st.
+static const char *model_points_with(AbiState &st, const Points *p, const Addresses *a, const UVW *d_uvws,
+                                     const JonesF64 *d_beam_jones, JonesF32 *d_vis_fb, JonesF64 *d_vis_fb_f64) {
     const int n = p->num_power_laws + p->num_curved_power_laws + p->num_lists;
-    if (n == 0) return nullptr;
     ModelCtx ctx;
-    if (const char *err = abi_ctx(a, d_uvws, d_beam_jones, n, ctx)) return err;
+    if (const char *err = abi_ctx(st, a, d_uvws, d_beam_jones, n, ctx)) return err;
     if (ctx.nbl == 0 || ctx.nchan == 0) return nullptr;
-    AbiState &st = abi_state();
     if (const char *err = gather_lmn(p->num_power_laws, p->power_law_lmns, p->num_curved_power_laws,
                                      p->curved_power_law_lmns, p->num_lists, p->list_lmns, st.lmn))
         return err;
@@ -430,9 +461,16 @@ extern "C" const char *sm_model_points_f64(const Points *p, const Addresses *a, 
                         p->power_law_sis, p->curved_power_law_fds, p->curved_power_law_sis, p->curved_power_law_qs,
                         p->list_fds};
     in.lmn = st.lmn.as<double4>();
-    SM_TRY(run_type(KIND_POINT, in, ctx, st.coh, d_vis_fb, d_vis_fb_f64, 0, nullptr, nullptr));
+    SM_TRY(run_type(KIND_POINT, in, ctx, st.coh, st.scratch, d_vis_fb, d_vis_fb_f64, 0, nullptr, nullptr));
     SM_TRY(hipStreamSynchronize(0));
     return nullptr;
+}
+
+extern "C" const char *sm_model_points_f64(const Points *p, const Addresses *a, const UVW *d_uvws,
+                                           const JonesF64 *d_beam_jones, JonesF32 *d_vis_fb, JonesF64 *d_vis_fb_f64) {
+    if (p == nullptr || a == nullptr) return "null argument";
+    if (p->num_power_laws + p->num_curved_power_laws + p->num_lists == 0) return nullptr;
+    return model_points_with(abi_state(), p, a, d_uvws, d_beam_jones, d_vis_fb, d_vis_fb_f64);
 }
 
 extern "C" const char *model_points(const Points *comps, const Addresses *a, const UVW *d_uvws,
@@ -446,10 +484,10 @@ extern "C" const char *sm_model_gaussians_f64(const Gaussians *g, const Addresse
     if (g == nullptr || a == nullptr) return "null argument";
     const int n = g->num_power_laws + g->num_curved_power_laws + g->num_lists;
     if (n == 0) return nullptr;
-    ModelCtx ctx;
-    if (const char *err = abi_ctx(a, d_uvws, d_beam_jones, n, ctx)) return err;
-    if (ctx.nbl == 0 || ctx.nchan == 0) return nullptr;
     AbiState &st = abi_state();
+    ModelCtx ctx;
+    if (const char *err = abi_ctx(st, a, d_uvws, d_beam_jones, n, ctx)) return err;
+    if (ctx.nbl == 0 || ctx.nchan == 0) return nullptr;
     if (const char *err = gather_lmn(g->num_power_laws, g->power_law_lmns, g->num_curved_power_laws,
                                      g->curved_power_law_lmns, g->num_lists, g->list_lmns, st.lmn))
         return err;
@@ -462,7 +500,7 @@ extern "C" const char *sm_model_gaussians_f64(const Gaussians *g, const Addresse
                         g->list_fds};
     in.lmn = st.lmn.as<double4>();
     in.shape = st.shape.as<double4>();
-    SM_TRY(run_type(KIND_GAUSSIAN, in, ctx, st.coh, d_vis_fb, d_vis_fb_f64, 0, nullptr, nullptr));
+    SM_TRY(run_type(KIND_GAUSSIAN, in, ctx, st.coh, st.scratch, d_vis_fb, d_vis_fb_f64, 0, nullptr, nullptr));
     SM_TRY(hipStreamSynchronize(0));
     return nullptr;
 }
@@ -480,10 +518,10 @@ extern "C" const char *sm_model_shapelets_f64(const Shapelets *s, const Addresse
     const int n = n_pl + n_cpl + n_list;
     if (n == 0) return nullptr;
     if (a->d_shapelet_basis_values == nullptr) return "d_shapelet_basis_values is null";
-    ModelCtx ctx;
-    if (const char *err = abi_ctx(a, d_uvws, d_beam_jones, n, ctx)) return err;
-    if (ctx.nbl == 0 || ctx.nchan == 0) return nullptr;
     AbiState &st = abi_state();
+    ModelCtx ctx;
+    if (const char *err = abi_ctx(st, a, d_uvws, d_beam_jones, n, ctx)) return err;
+    if (ctx.nbl == 0 || ctx.nchan == 0) return nullptr;
     if (const char *err = gather_lmn(n_pl, s->power_law_lmns, n_cpl, s->curved_power_law_lmns, n_list, s->list_lmns,
                                      st.lmn))
         return err;
@@ -534,7 +572,7 @@ extern "C" const char *sm_model_shapelets_f64(const Shapelets *s, const Addresse
     in.coeff_range = st.coeff_range.as<int2>();
     in.coeffs = st.coeffs.as<ShapeletCoeff>();
     in.coeff_count = (int)off;
-    SM_TRY(run_type(KIND_SHAPELET, in, ctx, st.coh, d_vis_fb, d_vis_fb_f64, 0, nullptr, nullptr));
+    SM_TRY(run_type(KIND_SHAPELET, in, ctx, st.coh, st.scratch, d_vis_fb, d_vis_fb_f64, 0, nullptr, nullptr));
     SM_TRY(hipStreamSynchronize(0));
     return nullptr;
 }
@@ -633,6 +671,11 @@ extern "C" const char *sm_analytic_beam(int32_t n_tiles, const uint32_t *delays,
     SM_TRY(launch_analytic_beam(n_tiles, dd.as<unsigned>(), dg.as<double>(), n_freqs, df.as<double>(), n_dirs, d_az,
                                 d_za, d_out, 0));
     SM_TRY(hipStreamSynchronize(0));
+    return nullptr;
+}
+
+extern "C" const char *sm_set_class_table_limit(int64_t bytes) {
+    sm::g_class_table_limit = bytes < 0 ? -1 : (long long)bytes;
     return nullptr;
 }
 

@@ -234,6 +234,7 @@ struct SmModeller {
     int npairs = 0;
     std::vector<double> h_tile_uvw;
     DevType types[3];
+    Scratch scratch;  // run_type's work buffers
     // timing: event pairs bracketing each direct-kernel launch, resolved lazily
     bool timing = false;
     struct Timed {
@@ -807,7 +808,7 @@ static const char *model_timestep_device(SmModeller *m, double lst, double latit
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (!fact_uniform && !fact_group) {
             if (const char *e = timed(ty, e0, e1)) return e;
-            SM_TRY(run_type(kinds[ty], in, ctx, d.coh, d_vis_fb, d_vis_fb_f64, 0, e0, e1));
+            SM_TRY(run_type(kinds[ty], in, ctx, d.coh, m->scratch, d_vis_fb, d_vis_fb_f64, 0, e0, e1));
             continue;
         }
         const int n = d.h.n();
@@ -832,7 +833,7 @@ static const char *model_timestep_device(SmModeller *m, double lst, double latit
             rest.nclass = 0;  // no baseline of the subset has a class-0 tile
             if (m->g0_nrest > 0) {
                 if (const char *e = timed(ty, e0, e1)) return e;
-                SM_TRY(run_type(kinds[ty], in, rest, d.coh, d_vis_fb, d_vis_fb_f64, 0, e0, e1));
+                SM_TRY(run_type(kinds[ty], in, rest, d.coh, m->scratch, d_vis_fb, d_vis_fb_f64, 0, e0, e1));
             }
             // Pairs with a class-0 tile: S here, J_r S J_0^H formed per chunk by the factored
             // path; UVWs of the factored tiles (zeros for padding).
@@ -995,6 +996,41 @@ extern "C" const char *sm_get_info(SmModeller *m, SmInfo *info) {
     info->kernel_ms_shapelets = m->ms_type[2];
     info->kernel_ms_factored = m->ms_type[3];
     info->num_factored_launches = m->launches_factored;
+    return nullptr;
+}
+
+extern "C" const char *sm_get_flux_params(SmModeller *m, int32_t comp_type, int32_t flux_type, int32_t index,
+                                          int32_t chan, JonesF64 *fd, double *si, double *q) {
+    if (m == nullptr || fd == nullptr) return "null argument";
+    if (comp_type < SM_POINT || comp_type > SM_SHAPELET) return "unknown component type";
+    const DevType &d = m->types[comp_type];
+    const HostType &h = d.h;
+    const JonesF64 *src = nullptr;
+    const double *d_si = nullptr, *d_q = nullptr;
+    size_t at = (size_t)index;
+    if (flux_type == SM_POWER_LAW) {
+        if (index < 0 || index >= h.n_pl) return "index outside the power-law bucket";
+        src = d.pl_fds.as<JonesF64>();
+        d_si = d.pl_sis.as<double>();
+    } else if (flux_type == SM_CURVED_POWER_LAW) {
+        if (index < 0 || index >= h.n_cpl) return "index outside the curved-power-law bucket";
+        src = d.cpl_fds.as<JonesF64>();
+        d_si = d.cpl_sis.as<double>();
+        d_q = d.cpl_qs.as<double>();
+    } else if (flux_type == SM_LIST) {
+        if (index < 0 || index >= h.n_list) return "index outside the list bucket";
+        if (chan < 0 || chan >= m->nchan) return "channel out of range";
+        src = d.list_fds.as<JonesF64>();
+        at = (size_t)chan * h.n_list + index;
+    } else {
+        return "unknown flux-density type";
+    }
+    // Read back from the device buffers the kernels use.
+    SM_TRY(hipMemcpy(fd, src + at, sizeof(JonesF64), hipMemcpyDeviceToHost));
+    if (si != nullptr) *si = 0.0;
+    if (q != nullptr) *q = 0.0;
+    if (si != nullptr && d_si != nullptr) SM_TRY(hipMemcpy(si, d_si + index, sizeof(double), hipMemcpyDeviceToHost));
+    if (q != nullptr && d_q != nullptr) SM_TRY(hipMemcpy(q, d_q + index, sizeof(double), hipMemcpyDeviceToHost));
     return nullptr;
 }
 

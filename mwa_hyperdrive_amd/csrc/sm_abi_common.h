@@ -59,9 +59,26 @@ struct ModelCtx {
 // 0: pick the fastest kernel for the case; 1: always the baseline-per-lane kernels.
 extern int g_direct_variant;
 
+// Device work buffers of run_type beyond the coherency table. The owner (a modeller handle,
+// or the calling thread for the kernel-level drop-ins) keeps them between calls and frees
+// them with itself.
+struct Scratch {
+    DevBuf accum;    // split component loop of a baseline subset: [nchan][nsub][8] FP64
+    DevBuf cls;      // shapelets: J_r S J_0^H per beam-row class
+    DevBuf prange, packed;  // shapelets: packed coefficients
+    DevBuf coh0;     // J_0 S J_0^H of beam-row class 0
+    DevBuf cseg;     // one segment's shared coherency (fallback of the row-by-column kernel)
+    DevBuf ccls;     // row-by-column kernel: J_r S J_0^H per beam-row class
+};
+
+// Largest per-class coherency table (bytes) run_type builds; above it, or when the
+// allocation fails, it takes the per-segment path. Negative (default): the device's free
+// memory less 1 GiB, read when the table is needed. Process-wide.
+extern long long g_class_table_limit;
+
 // Coherency table + direct kernel for one type; ev0/ev1 (nullable) bracket the direct kernel.
-hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf &coh, JonesF32 *vis32,
-                    JonesF64 *vis64, hipStream_t stream, hipEvent_t ev0, hipEvent_t ev1);
+hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf &coh, Scratch &scratch,
+                    JonesF32 *vis32, JonesF64 *vis64, hipStream_t stream, hipEvent_t ev0, hipEvent_t ev1);
 
 double4 host_gaussian_shape(GaussianParams g);
 double4 host_shapelet_shape(GaussianParams g);

@@ -122,12 +122,21 @@ int pad_channels(int nchan);
 // Pads a frequency list to whole channel chunks by continuing its last step.
 void pad_freqs(const double *freqs, int nchan, double *out_padded);
 
-// A grow-only device buffer.
+// A grow-only device buffer. A borrowed buffer views memory it does not own (set up with
+// borrow()): it never allocates or frees, and reserve() beyond its capacity fails with
+// hipErrorOutOfMemory.
 struct DevBuf {
     void *ptr = nullptr;
     size_t bytes = 0;
+    bool borrowed = false;
     hipError_t reserve(size_t n);
     void release();
+    void borrow(void *p, size_t n) {
+        release();
+        ptr = p;
+        bytes = n;
+        borrowed = true;
+    }
     ~DevBuf() { release(); }
     template <class T> T *as() const { return static_cast<T *>(ptr); }
 };

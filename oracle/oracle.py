@@ -60,6 +60,7 @@ def lib():
         L.orc_analytic_beam_jones.argtypes = [d, d, d, p, p, p]
         L.orc_model_timestep.argtypes = [C.POINTER(OrcObs), p, p, i32, i32, p, p, i32]
         L.orc_model_timestep_slice.argtypes = [C.POINTER(OrcObs), p, p, i32, i32, i32, p, p, i32]
+        L.orc_model_timestep_baselines.argtypes = [C.POINTER(OrcObs), p, p, i32, i32, i32, p, i32, p, p, i32]
         L.orc_model_timestep_autos.argtypes = [C.POINTER(OrcObs), p, p, i32, p, p, i32]
         L.orc_unique_tiles.argtypes = [i32, p, i32, p, p, p]
         L.orc_unique_tiles.restype = i32
@@ -140,6 +141,25 @@ def model_timestep_slice(obs: Obs, srclist, chan_start: int, chan_count: int, nu
                                     packed.num_sources, chan_start, chan_count, _ptr(v32), _ptr(v64), nt)
     if rc != 0:
         raise RuntimeError(f"orc_model_timestep_slice failed ({rc})")
+    return v32, v64
+
+
+def model_timestep_baselines(obs: Obs, srclist, chan_start: int, chan_count: int, baselines, num_threads: int = 0):
+    """Channels [chan_start, chan_start + chan_count) of the listed baselines only:
+    (vis_f32, vis_f64) of shape [chan_count][len(baselines)][8]."""
+    L = lib()
+    packed = pack_source_list(srclist)
+    bls = np.ascontiguousarray(np.asarray(baselines, np.int32))
+    shape = (chan_count, len(bls), 8)
+    v32 = np.zeros(shape, np.float32)
+    v64 = np.zeros(shape, np.float64)
+    co = obs.c()
+    nt = num_threads or min(16, os.cpu_count() or 1)
+    rc = L.orc_model_timestep_baselines(C.byref(co), C.cast(packed.comps, C.c_void_p),
+                                        C.cast(packed.offsets, C.c_void_p), packed.num_sources, chan_start, chan_count,
+                                        _ptr(bls), len(bls), _ptr(v32), _ptr(v64), nt)
+    if rc != 0:
+        raise RuntimeError(f"orc_model_timestep_baselines failed ({rc})")
     return v32, v64
 
 

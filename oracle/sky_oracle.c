@@ -584,7 +584,8 @@ static const double GAUSSIAN_EXP_CONST = -(FRAC_PI_2 * FRAC_PI_2) / LN_2; /* src
 /* The component folds of model_points / model_gaussians / model_shapelets
  * (src/model/cpu.rs:303-350, 426-492, 592-723) for channels [f_start, f_start+f_count). */
 static void model_type(const modeller *m, int ty, int32_t f_start, int32_t f_count, int32_t vis_nf,
-                       int32_t vis_f0, float *vis_f32, double *vis_f64, int32_t num_threads) {
+                       int32_t vis_f0, float *vis_f32, double *vis_f64, int32_t num_threads,
+                       const int32_t *bl_list, int32_t n_bl_list) {
     const type_list *tl = &m->types[ty];
     if (tl->n == 0) return;
     const OrcObs *obs = m->obs;
@@ -592,14 +593,16 @@ static void model_type(const modeller *m, int ty, int32_t f_start, int32_t f_cou
     const double *table = (ty == ORC_SHAPELET) ? orc_shapelet_basis_table_cached() : NULL;
     const double shapelet_const = SQRT_FRAC_PI_SQ_2_LN_2 / SBF_DX;
     int32_t n = tl->n;
-    int32_t nbl = m->num_bl;
+    /* Output rows: every baseline, or (bl_list non-null) the listed baselines in list order. */
+    int32_t nbl = bl_list ? n_bl_list : m->num_bl;
     (void)vis_nf;
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 16) num_threads(num_threads > 0 ? num_threads : 1)
 #else
     (void)num_threads;
 #endif
-    for (int32_t b = 0; b < nbl; b++) {
+    for (int32_t row = 0; row < nbl; row++) {
+        int32_t b = bl_list ? bl_list[row] : row;
         int32_t t1 = m->tile_map[m->unflagged_to_total[m->bl_t1[b]]];
         int32_t t2 = m->tile_map[m->unflagged_to_total[m->bl_t2[b]]];
         c64 *env = (ty == ORC_SHAPELET) ? (c64 *)malloc(sizeof(c64) * (size_t)n) : NULL;
@@ -674,7 +677,7 @@ static void model_type(const modeller *m, int ty, int32_t f_start, int32_t f_cou
                 }
                 jadd_inplace(&accum, t);
             }
-            size_t o = ((size_t)(f - vis_f0) * nbl + b) * 8;
+            size_t o = ((size_t)(f - vis_f0) * nbl + row) * 8;
             for (int k = 0; k < 4; k++) {
                 vis_f32[o + 2 * k] += (float)accum.j[k].re;
                 vis_f32[o + 2 * k + 1] += (float)accum.j[k].im;
@@ -721,7 +724,7 @@ int orc_model_timestep(const OrcObs *obs, const OrcComponent *comps, const int32
     build_modeller(&m, obs, comps, source_offsets, num_sources, 0, obs->num_freqs);
     for (int ty = 0; ty < 3; ty++)
         if (comp_type_mask & (1 << ty)) model_type(&m, ty, 0, obs->num_freqs, obs->num_freqs, 0, vis_f32, vis_f64,
-                                                   num_threads);
+                                                   num_threads, NULL, 0);
     mask_pols(vis_f32, vis_f64, (size_t)obs->num_freqs * m.num_bl, obs->pols);
     free_modeller(&m);
     return 0;
@@ -735,7 +738,27 @@ int orc_model_timestep_slice(const OrcObs *obs, const OrcComponent *comps, const
     modeller m;
     build_modeller(&m, obs, comps, source_offsets, num_sources, chan_start, chan_count);
     for (int ty = 0; ty < 3; ty++)
-        model_type(&m, ty, chan_start, chan_count, chan_count, chan_start, vis_f32, vis_f64, num_threads);
+        model_type(&m, ty, chan_start, chan_count, chan_count, chan_start, vis_f32, vis_f64, num_threads, NULL, 0);
+    free_modeller(&m);
+    return 0;
+}
+
+int orc_model_timestep_baselines(const OrcObs *obs, const OrcComponent *comps, const int32_t *source_offsets,
+                                 int32_t num_sources, int32_t chan_start, int32_t chan_count,
+                                 const int32_t *baselines, int32_t num_listed, float *vis_f32, double *vis_f64,
+                                 int32_t num_threads) {
+    if (obs->num_unflagged_tiles < 2 || chan_start < 0 || chan_count < 1 || chan_start + chan_count > obs->num_freqs ||
+        baselines == NULL || num_listed < 1)
+        return 1;
+    int32_t nt = obs->num_unflagged_tiles;
+    int32_t nbl = nt * (nt - 1) / 2;
+    for (int32_t k = 0; k < num_listed; k++)
+        if (baselines[k] < 0 || baselines[k] >= nbl) return 1;
+    modeller m;
+    build_modeller(&m, obs, comps, source_offsets, num_sources, chan_start, chan_count);
+    for (int ty = 0; ty < 3; ty++)
+        model_type(&m, ty, chan_start, chan_count, chan_count, chan_start, vis_f32, vis_f64, num_threads, baselines,
+                   num_listed);
     free_modeller(&m);
     return 0;
 }

@@ -98,6 +98,14 @@ int orc_model_timestep_slice(const OrcObs *obs, const OrcComponent *comps, const
                              int32_t num_sources, int32_t chan_start, int32_t chan_count, float *vis_f32,
                              double *vis_f64, int32_t num_threads);
 
+/* Same as the slice, for the listed baselines only: vis [chan_count][num_listed][8], row k
+ * holding baseline baselines[k] (the numbering of src/math/mod.rs:96-109). Used to check
+ * full-size skies on a bounded sample of baselines. */
+int orc_model_timestep_baselines(const OrcObs *obs, const OrcComponent *comps, const int32_t *source_offsets,
+                                 int32_t num_sources, int32_t chan_start, int32_t chan_count,
+                                 const int32_t *baselines, int32_t num_listed, float *vis_f32, double *vis_f64,
+                                 int32_t num_threads);
+
 int orc_model_timestep_autos(const OrcObs *obs, const OrcComponent *comps, const int32_t *source_offsets,
                              int32_t num_sources, float *vis_f32, double *vis_f64, int32_t num_threads);
 

@@ -247,3 +247,56 @@ def test_dropin_autos_vs_oracle():
     obs = orc.Obs(xyzs, freqs, pc, 0.0, synthetic.MWA_LAT_RAD)
     o32, o64 = orc.model_autos(obs, SourceList([("s", Source(b.order))]))
     assert rel_err(vis.cpu().numpy(), o64) < 1e-7
+
+
+def test_dropins_from_two_threads():
+    """The kernel-level drop-ins keep their work buffers per calling thread: two threads that
+    model different skies into their own buffers at the same time get the results each gets
+    alone, bit for bit (the reference's SkyModellerGpu owns its buffers per modeller,
+    src/model/gpu.rs:150-519)."""
+    import threading
+
+    rng = np.random.default_rng(19)
+    nt = 24
+    xyzs = synthetic.tiles(rng, nt, 900.0)
+    freqs = 167e6 + 40e3 * np.arange(33)
+    pc = synthetic.PHASE_CENTRE
+    lst = 0.1
+    nbl = nt * (nt - 1) // 2
+    jobs = []
+    for seed in (1, 2):
+        comps = make_sky(np.random.default_rng(100 + seed), 20 * seed, freqs, "point")
+        b = Buckets(comps, freqs, pc, xyzs, lst)
+        n = b.n
+        s = _lib.Points(n["pl"], b.g("pl_lmn"), b.g("pl_fds"), b.g("pl_sis"), n["cpl"], b.g("cpl_lmn"),
+                        b.g("cpl_fds"), b.g("cpl_sis"), b.g("cpl_qs"), n["list"], b.g("list_lmn"), b.g("list_fds"))
+        jones = np.zeros((1, 1, len(b.order), 8))
+        jones[..., 0] = jones[..., 6] = 1.0
+        from oracle import oracle as orc
+        d_uvw = dt(orc.cross_uvws(xyzs, lst - pc[0], pc[1]))
+        a, keep = addresses(freqs, nbl, 1, np.zeros(nt), np.zeros(len(freqs)), np.arange(nt))
+        jobs.append(dict(b=b, s=s, a=a, keep=keep, d_beam=dt(jones), d_uvw=d_uvw))
+
+    def run(job, out):
+        out64 = torch.zeros((len(freqs), nbl, 8), dtype=torch.float64, device=dev)
+        out32 = torch.zeros((len(freqs), nbl, 8), dtype=torch.float32, device=dev)
+        for _ in range(3):
+            _lib.call("sm_model_points_f64", C.byref(job["s"]), C.byref(job["a"]), p(job["d_uvw"]), p(job["d_beam"]),
+                      p(out32), p(out64))
+        out.append((out32, out64))
+
+    alone = []
+    for job in jobs:
+        run(job, alone)
+    torch.cuda.synchronize()
+    together = [[], []]
+    threads = [threading.Thread(target=run, args=(job, res)) for job, res in zip(jobs, together)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    torch.cuda.synchronize()
+    for (a32, a64), res in zip(alone, together):
+        (t32, t64), = res
+        np.testing.assert_array_equal(t64.cpu().numpy(), a64.cpu().numpy())
+        np.testing.assert_array_equal(t32.cpu().numpy(), a32.cpu().numpy())

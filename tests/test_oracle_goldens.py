@@ -225,6 +225,34 @@ def test_azel_closed_forms():
     assert abs(az - math.pi) < 1e-12
 
 
+def test_azel_off_meridian_closed_forms():
+    """Away from the meridian (ERFA hd2ae, which marlu's HADec::to_azel calls): a source east
+    of the meridian (HA < 0) has az in (0, pi), west (HA > 0) in (pi, 2 pi). For an observer
+    on the equator a source on the celestial equator at HA = -h stands due east at elevation
+    pi/2 - h, and at HA = +h due west. At the south pole el = -Dec and az = -HA. Each fails if
+    the sign of y = -sin(HA) cos(Dec) is flipped or two axes are swapped."""
+    for ha in (-2.0, -0.7, -0.05):
+        for dec in (MWA_LAT_RAD - 0.3, MWA_LAT_RAD + 0.2, 0.1):
+            az, el = orc.azel(ha, dec, MWA_LAT_RAD)
+            assert 0.0 < az < math.pi, (ha, dec, az)
+            az_w, el_w = orc.azel(-ha, dec, MWA_LAT_RAD)
+            assert math.pi < az_w < 2 * math.pi
+            # East-west mirror symmetry: same elevation, az -> 2 pi - az.
+            assert abs(el_w - el) < 1e-14 and abs(az_w - (2 * math.pi - az)) < 1e-13
+    for h in (0.2, 0.9, 1.4):
+        az, el = orc.azel(-h, 0.0, 0.0)
+        assert abs(az - math.pi / 2) < 1e-14 and abs(el - (math.pi / 2 - h)) < 1e-14
+        az, el = orc.azel(h, 0.0, 0.0)
+        assert abs(az - 3 * math.pi / 2) < 1e-14 and abs(el - (math.pi / 2 - h)) < 1e-14
+    # Observer at the south pole, where the south celestial pole is the zenith: el = -Dec and
+    # az = -HA (mod 2 pi).
+    lat = -math.pi / 2
+    for ha, dec in ((0.3, -0.5), (-1.1, -0.2), (2.5, -1.0)):
+        az, el = orc.azel(ha, dec, lat)
+        assert abs(el - (-dec)) < 1e-14
+        assert abs(math.cos(az) - math.cos(ha)) < 1e-12 and abs(math.sin(az) + math.sin(ha)) < 1e-12
+
+
 def test_analytic_beam_closed_forms():
     """Own beam model: identity at zenith for an ideal tile; a tile with every dipole dead is
     zero; the steering delays move the peak; the array factor of 16 equal dipoles at zenith is

@@ -53,23 +53,81 @@ def cpu_model_name() -> str:
     return platform.processor() or "unknown"
 
 
-def cpu_baseline(obs, chans: int, threads: int) -> dict:
+def native_oracle():
+    """The oracle built for this host's CPU (-march=native, as the CPU baseline is specified)
+    into a private temporary folder; the tests use the portable build in oracle/."""
+    import subprocess
+    import tempfile
+
     from oracle import oracle as orc
 
+    out = Path(tempfile.mkdtemp(prefix="skyoracle_native_")) / "libskyoracle.so"
+    src = ROOT / "oracle" / "sky_oracle.c"
+    cmd = ["gcc", "-O3", "-march=native", "-fPIC", "-fopenmp", "-ffp-contract=off", "-fno-fast-math", "-shared",
+           "-o", str(out), str(src), "-lm"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        print(f"[bench] native oracle build failed, using the portable build:\n{r.stderr}", file=sys.stderr)
+        return orc, "portable (-march=x86-64-v3)"
+    orc._LIB_PATH = out
+    orc._lib = None
+    orc.lib()
+    return orc, "-O3 -march=native -ffp-contract=off"
+
+
+def cpu_baseline(obs, chans: int, threads: int) -> dict:
+    """The oracle (the C restatement of SkyModellerCpu, same loop nest, f64) on the host's
+    cores: C1 in full, and a bounded slice of C2 (the first `chans` channels of one timestep,
+    all components), which is the reported value."""
+    from mwa_hyperdrive_amd import synthetic
+
+    orc, flags = native_oracle()
+    c1 = synthetic.config(1)
+    o1 = orc.Obs(c1.xyzs, c1.freqs, c1.phase_centre, float(c1.lsts[0]), c1.latitude)
+    t0 = time.perf_counter()
+    orc.model_timestep(o1, c1.srclist, num_threads=threads)
+    dt1 = time.perf_counter() - t0
+    work1 = c1.num_components * c1.num_baselines * len(c1.freqs)
     o = orc.Obs(obs.xyzs, obs.freqs, obs.phase_centre, float(obs.lsts[0]), obs.latitude, beam=obs.beam,
                 delays=obs.delays, gains=obs.gains)
     t0 = time.perf_counter()
     orc.model_timestep_slice(o, obs.srclist, 0, chans, threads)
     dt = time.perf_counter() - t0
     work = obs.num_components * obs.num_baselines * chans
+    print(f"[bench] CPU baseline: C1 in full {dt1:.2f} s, C2 slice {dt:.1f} s", file=sys.stderr, flush=True)
     return {
         "value": work / dt / 1e9,
         "unit": "Gcomp·vis/s",
         "cores": threads,
         "kind": "port",
         "sample": f"C2 timestep 0, channels 0-{chans - 1}, all {obs.num_components} components "
-                  f"({work:.3e} comp.vis, {dt:.1f} s) on {cpu_model_name()}",
+                  f"({work:.3e} comp.vis, {dt:.1f} s)",
+        "c1_full": {"comp_vis": work1, "seconds": dt1, "gcompvis_per_s": work1 / dt1 / 1e9},
+        "host": {"cpu_model": cpu_model_name(), "nproc": os.cpu_count(),
+                 "affinity_cpus": len(os.sched_getaffinity(0)), "threads_used": threads, "oracle_flags": flags},
     }
+
+
+def host_entry_c2(obs, timesteps: int = 2) -> dict:
+    """C2 timesteps through the host-buffer entry (sm_model_timestep_with): the visibilities
+    go host -> device and back (200 MB each way per timestep in f32), as the reference's
+    SkyModellerGpu::model_timestep_with does (src/model/gpu.rs:1476-1496)."""
+    import numpy as np
+
+    from mwa_hyperdrive_amd.modeller import AnalyticBeam, SkyModeller
+
+    m = SkyModeller(AnalyticBeam(obs.delays, obs.gains), obs.srclist, 0, obs.xyzs, obs.freqs, [],
+                    obs.phase_centre, 0.0, obs.latitude)
+    vis = np.zeros((len(obs.freqs), obs.num_baselines, 8), np.float32)
+    m.model_timestep_with(float(obs.lsts[0]), obs.latitude, vis)  # warm-up (allocations)
+    t0 = time.perf_counter()
+    for t in range(timesteps):
+        m.model_timestep_with(float(obs.lsts[t]), obs.latitude, vis)
+    dt = (time.perf_counter() - t0) / timesteps
+    work = obs.num_components * obs.num_baselines * len(obs.freqs)
+    del m
+    return {"ms_per_timestep": dt * 1e3, "gcompvis_per_s": work / dt / 1e9,
+            "vis_bytes_each_way": vis.nbytes, "timesteps": timesteps}
 
 
 FLOP_PER_COMPVIS_GAUSSIAN = FLOP_PER_COMPVIS_POINT + 21.0  # (kx, ky) 6, quadratic form 5, x K 1, exp 1, scale 8
@@ -405,17 +463,24 @@ def main() -> int:
             "shapelets": info["kernel_ms_shapelets"] / args.steps,
         },
     }
+    if traffic is not None:
+        out["roofline"]["hbm_gbs"] = traffic / (per_kernel[dom]["avg_launch_ms"] * 1e-3) / 1e9
+        out["roofline"]["hbm_frac"] = out["roofline"]["hbm_gbs"] / PEAK_HBM_GBS
     del m, vis
     torch.cuda.empty_cache()
+    out["host_entry_c2"] = host_entry_c2(obs)
+    print(f"[bench] host entry: {out['host_entry_c2']['ms_per_timestep']:.1f} ms per C2 timestep",
+          file=sys.stderr, flush=True)
     if not args.no_split_precision:
         out["split_precision"] = split_precision()
         out["combined_c5"] = combined_c5()
     if not args.no_surrogate:
         out["surrogate"] = surrogate_bench(obs)
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        print(f"[bench] CPU baseline: oracle on {threads} threads, {args.cpu_chans} channels", file=sys.stderr,
-              flush=True)
+        # Every CPU this process may run on, up to the 16 a one-GPU box allots per GPU.
+        threads = min(16, len(os.sched_getaffinity(0)))
+        print(f"[bench] CPU baseline: oracle on {threads} threads ({cpu_model_name()}, nproc {os.cpu_count()}), "
+              f"{args.cpu_chans} channels", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(obs, args.cpu_chans, threads)
     else:
         out["cpu_baseline"] = None

@@ -96,6 +96,13 @@ static hipError_t launch_subset(Kind kind, bool per_tile, bool even, DirectArgs 
 
 long long g_class_table_limit = -1;
 
+// An FP32 copy of n FP64 values in buf.
+static hipError_t to_f32(DevBuf &buf, size_t n, const double *src, hipStream_t stream) {
+    hipError_t e = buf.reserve(sizeof(float) * (n > 0 ? n : 1));
+    if (e != hipSuccess) return e;
+    return launch_to_f32(n, src, buf.as<float>(), stream);
+}
+
 // Reserves a per-class coherency table of `bytes` in buf if it is within the limit and the
 // allocation succeeds; false sends the caller to its per-segment path. A failed allocation
 // is cleared from the runtime's last-error state, so it does not surface later.
@@ -150,6 +157,8 @@ hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf
         beam.class_of_row = ctx.d_class_of_row;
         beam.row0 = ctx.class_row0;
     }
+    const size_t coh_n = (size_t)n * ctx.nchan_pad * 8;
+    const bool f32 = ctx.fp32 && kind != KIND_POINT && ctx.d_bl_subset == nullptr && g_direct_variant == 0;
     // Shapelet coefficients in the kernels' packed form.
     DevBuf &prange = scratch.prange, &packed = scratch.packed;
     const int4 *d_prange = nullptr;
@@ -182,6 +191,7 @@ hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf
     a.vis64 = vis64;
     a.bl_subset = ctx.d_bl_subset;
     a.nsub = ctx.nsub;
+    a.chunks = ctx.chunks;
     const bool g0_tiled = per_tile && ctx.d_bl_subset == nullptr && g_direct_variant == 0 && kind != KIND_SHAPELET &&
                           ctx.even && ctx.g0_n >= 2 && ctx.d_g0_uvw != nullptr && ctx.class_row0 >= 0;
     if (g0_tiled) {
@@ -193,6 +203,7 @@ hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf
         BeamDesc b0 = ctx.beam;
         b0.uniform_row = ctx.class_row0;
         e = launch_coherency(n, ctx.nchan, ctx.nchan_pad, ctx.d_freqs_pad, in.fb, b0, true, coh0.as<double>(), stream);
+        if (e == hipSuccess && f32) e = to_f32(scratch.coh0_32, coh_n, coh0.as<double>(), stream);
         if (e != hipSuccess) return e;
         TiledArgs t;
         t.ntiles = ctx.g0_n;
@@ -208,10 +219,12 @@ hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf
         t.lmn = in.lmn;
         t.shape = in.shape;
         t.coh = coh0.as<double>();
+        if (f32) t.coh32 = scratch.coh0_32.as<float>();
         t.vis32 = vis32;
         t.vis64 = vis64;
         t.tile_map = ctx.d_g0_tiles;
         t.ntiles_bl = ctx.ntiles;
+        t.chunks = ctx.chunks;
         if (ev0 != nullptr) (void)hipEventRecord(ev0, stream);
         e = launch_direct_tiled(kind, t, stream);
         // Pairs of a class-r tile with a class-0 tile. With a table of J_r S J_0^H for every
@@ -226,6 +239,7 @@ hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf
         if (rect) {
             e = launch_class_coherency(ctx.nclass, n, ctx.nchan, ctx.nchan_pad, coh.as<double>(), ctx.beam,
                                        ctx.d_class_row, ccls.as<double>(), stream);
+            if (e == hipSuccess && f32) e = to_f32(scratch.ccls32, (size_t)ctx.nclass * coh_n, ccls.as<double>(), stream);
             if (e == hipSuccess) {
                 RectArgs r;
                 r.n_row = ctx.n_other;
@@ -243,8 +257,10 @@ hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf
                 r.lmn = in.lmn;
                 r.shape = in.shape;
                 r.coh_cls = ccls.as<double>();
+                if (f32) r.coh_cls32 = scratch.ccls32.as<float>();
                 r.vis32 = vis32;
                 r.vis64 = vis64;
+                r.chunks = ctx.chunks;
                 e = launch_direct_rect(kind, r, stream);
             }
         }
@@ -295,8 +311,14 @@ hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf
         t.lmn = in.lmn;
         t.shape = in.shape;
         t.coh = coh.as<double>();
+        if (f32) {
+            e = to_f32(scratch.coh32, coh_n, coh.as<double>(), stream);
+            if (e != hipSuccess) return e;
+            t.coh32 = scratch.coh32.as<float>();
+        }
         t.vis32 = vis32;
         t.vis64 = vis64;
+        t.chunks = ctx.chunks;
         e = launch_direct_tiled(kind, t, stream);
     } else if (g_direct_variant != 1 && kind == KIND_SHAPELET) {
         ShapeletChanArgs s;
@@ -317,6 +339,15 @@ hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf
         s.beam = beam;
         s.vis32 = vis32;
         s.vis64 = vis64;
+        if (f32 && ctx.d_sbf32 != nullptr) {
+            e = to_f32(scratch.coh32, coh_n, coh.as<double>(), stream);
+            if (e == hipSuccess && beam.coh_cls != nullptr)
+                e = to_f32(scratch.cls32, (size_t)ctx.nclass * coh_n, beam.coh_cls, stream);
+            if (e != hipSuccess) return e;
+            s.coh32 = scratch.coh32.as<float>();
+            if (beam.coh_cls != nullptr) s.coh_cls32 = scratch.cls32.as<float>();
+            s.sbf32 = ctx.d_sbf32;
+        }
         e = launch_shapelet_chan(per_tile, s, stream);
     } else {
         e = launch_direct(kind, per_tile, ctx.even, a, stream);

@@ -225,7 +225,7 @@ __global__ __launch_bounds__(256) void k_direct(const DirectArgs a, int gx, int 
     // Component range: all, or one of csplit ranges (subset mode).
     int c_lo = 0, c_hi = a.ncomp;
     if (SPLIT) {
-        const int gy = (a.nchan + NCH - 1) / NCH;
+        const int gy = chunk_count(a.chunks, NCH, a.nchan);
         const int split = chunk / gy;
         chunk -= split * gy;
         const int per = (a.ncomp + a.csplit - 1) / a.csplit;
@@ -237,7 +237,8 @@ __global__ __launch_bounds__(256) void k_direct(const DirectArgs a, int gx, int 
     const int bl_raw = bx * 256 + (int)threadIdx.x;
     const int li = bl_raw < nlanes ? bl_raw : nlanes - 1;
     const int bl = a.bl_subset != nullptr ? a.bl_subset[li] : li;
-    const int f0 = chunk * NCH;
+    int f0, cnt;
+    chunk_range<NCH>(a.chunks, chunk, a.nchan, f0, cnt);
 
     const double *__restrict__ coh = a.coh;
     const double4 *__restrict__ lmn4 = a.lmn;
@@ -344,7 +345,7 @@ __global__ __launch_bounds__(256) void k_direct(const DirectArgs a, int gx, int 
 #pragma unroll
         for (int k = 0; k < NCH; ++k) {
             const int f = f0 + k;
-            if (f >= a.nchan) break;
+            if (k >= cnt) break;
             double *dst = a.accum + ((size_t)f * a.nsub + li) * 8;
 #pragma unroll
             for (int e = 0; e < 8; ++e) unsafeAtomicAdd(&dst[e], acc[k][e]);
@@ -354,7 +355,7 @@ __global__ __launch_bounds__(256) void k_direct(const DirectArgs a, int gx, int 
 #pragma unroll
     for (int k = 0; k < NCH; ++k) {
         const int f = f0 + k;
-        if (f >= a.nchan) break;
+        if (k >= cnt) break;
         const size_t o = (size_t)f * a.nbl + bl;
         float *v32 = reinterpret_cast<float *>(&a.vis32[o]);
 #pragma unroll
@@ -379,7 +380,7 @@ template <int KIND, bool PT>
 static hipError_t launch_direct_kind(const DirectArgs &a, bool even, hipStream_t stream) {
     constexpr int NCH = kDirectNch<KIND, PT>;
     const int gx = ((a.bl_subset != nullptr ? a.nsub : a.nbl) + 255) / 256;
-    const int gy = (a.nchan + NCH - 1) / NCH;
+    const int gy = chunk_count(a.chunks, NCH, a.nchan);
     const int total = gx * gy * a.csplit;
     // The split form is a separate instantiation so that the common kernels compile as before.
     if (a.csplit > 1 && even)
@@ -391,6 +392,20 @@ static hipError_t launch_direct_kind(const DirectArgs &a, bool even, hipStream_t
     else
         hipLaunchKernelGGL((k_direct<KIND, PT, false, false, NCH>), dim3(total), dim3(256), 0, stream, a, gx, total);
     return hipGetLastError();
+}
+
+std::vector<int2> even_chunks(const double *freqs, int nchan, int width) {
+    std::vector<int2> out;
+    for (int f = 0; f < nchan;) {
+        int cnt = nchan - f >= 2 ? 2 : 1;
+        if (cnt == 2) {
+            const double d = freqs[f + 1] - freqs[f];
+            while (cnt < width && f + cnt < nchan && freqs[f + cnt] == freqs[f] + (double)cnt * d) ++cnt;
+        }
+        out.push_back(make_int2(f, cnt));
+        f += cnt;
+    }
+    return out;
 }
 
 // Whether every chunk's channels are exactly f0 + k (f1 - f0); otherwise each channel gets

@@ -34,14 +34,67 @@ constexpr int kTiledNchGaussian = 8;
 static_assert(kTiledNchPoint <= SM_NCH && kTiledNchGaussian <= SM_NCH,
               "the padded channel count covers whole chunks of SM_NCH channels");
 
-template <int KIND, int NCH>
+// The kernels below come in two arithmetic types T. double: the FP64 direct kernels. float:
+// the precision path's kernels, in which the phases, their range reduction mod 2 pi, the table
+// positions of shapelets and the Gaussian exponents are still formed in FP64, and everything
+// after (phasor products, envelopes, coherency products, accumulation) runs in FP32.
+template <class T> struct Vec4;
+template <> struct Vec4<double> { using type = double4; };
+template <> struct Vec4<float> { using type = float4; };
+
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return fma(a, b, c); }
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// e^x of an FP64 argument: FP64 exp, or the hardware base-2 exponential in FP32 (an argument
+// below the FP32 range gives 0).
+__device__ __forceinline__ void exp_of(double x, double &out) { out = exp(x); }
+__device__ __forceinline__ void exp_of(double x, float &out) {
+    out = __builtin_amdgcn_exp2f((float)(x * 1.4426950408889634));
+}
+
+// sin and cos of an FP64 phase in the kernel's arithmetic. FP32: x is reduced mod pi/2 in FP64
+// exactly as fast_sincos does, and the reduced |r| <= pi/4 goes through the Taylor series of
+// sin to r^9 and of cos to r^10 in FP32 (truncation below 2e-9, under the FP32 rounding).
+__device__ __forceinline__ void sincos_phase(double x, double &s, double &c) { fast_sincos(x, s, c); }
+__device__ __forceinline__ void sincos_phase(double x, float &s, float &c) {
+    const double k = rint(x * 0.6366197723675814);
+    double r = fma(-k, 1.5707963267948966, x);
+    r = fma(-k, 6.123233995736766e-17, r);
+    const float rf = (float)r, r2 = rf * rf;
+    // coefficients (-1)^n / (2n+1)! and (-1)^n / (2n)!
+    float ps = 2.7557319e-06f;
+    ps = __builtin_fmaf(ps, r2, -1.9841270e-04f);
+    ps = __builtin_fmaf(ps, r2, 8.3333333e-03f);
+    ps = __builtin_fmaf(ps, r2, -1.6666667e-01f);
+    const float sn = __builtin_fmaf(rf * r2, ps, rf);
+    float pc = -2.7557319e-07f;
+    pc = __builtin_fmaf(pc, r2, 2.4801587e-05f);
+    pc = __builtin_fmaf(pc, r2, -1.3888889e-03f);
+    pc = __builtin_fmaf(pc, r2, 4.1666667e-02f);
+    pc = __builtin_fmaf(pc, r2, -0.5f);
+    const float cn = __builtin_fmaf(r2, pc, 1.0f);
+    const int q = (int)(long long)k & 3;
+    const float a = (q & 1) ? cn : sn;  // sin x = sin r, cos r, -sin r, -cos r for q = 0..3
+    const float b = (q & 1) ? sn : cn;  // cos x = cos r, -sin r, -cos r, sin r
+    s = (q & 2) ? -a : a;
+    c = ((q + 1) & 2) ? -b : b;
+}
+
+template <class T> __device__ __forceinline__ const T *pick(const double *d, const float *f);
+template <> __device__ __forceinline__ const double *pick<double>(const double *d, const float *) { return d; }
+template <> __device__ __forceinline__ const float *pick<float>(const double *, const float *f) { return f; }
+
+template <int KIND, int NCH, class T>
 __global__ __launch_bounds__(256) void k_direct_tiled(const TiledArgs a) {
-    __shared__ double4 s_ph[kCT][32];
+    using V4 = typename Vec4<T>::type;
+    __shared__ V4 s_ph[kCT][32];
     __shared__ double s_uvw[32][3];
     const int total = a.npairs * a.nchunks;
     int b = blockIdx.x;
     if ((total & 7) == 0) b = (b & 7) * (total >> 3) + (b >> 3);
     const int chunk = b / a.npairs;
+    int f0, cnt;
+    chunk_range<NCH>(a.chunks, chunk, a.nchan, f0, cnt);
     const int4 pr = a.pairs[b - chunk * a.npairs];
     const int tid = threadIdx.x, r = tid >> 4, cq = tid & 15;
     if (tid < 32) {
@@ -83,70 +136,74 @@ __global__ __launch_bounds__(256) void k_direct_tiled(const TiledArgs a) {
     const double bu = s_uvw[sp][0] - s_uvw[sq][0];
     const double bv = s_uvw[sp][1] - s_uvw[sq][1];
 
-    const int f0 = chunk * NCH;
     const double *__restrict__ freqs = a.freqs;
     const double F0 = freqs[f0];
     const double DF = freqs[f0 + 1] - F0;
     const double f0c = F0 / SM_VEL_C;
     const double dfc = DF / SM_VEL_C;
-    const double *__restrict__ coh = a.coh;
+    const T *__restrict__ coh = pick<T>(a.coh, a.coh32);
     const double4 *__restrict__ lmn4 = a.lmn;
     const double4 *__restrict__ shape = a.shape;
 
-    double acc[NCH][8];
+    T acc[NCH][8];
 #pragma unroll
     for (int k = 0; k < NCH; ++k)
 #pragma unroll
-        for (int e = 0; e < 8; ++e) acc[k][e] = 0.0;
+        for (int e = 0; e < 8; ++e) acc[k][e] = T(0);
 
     for (int c0 = 0; c0 < a.ncomp; c0 += kCT) {
         const int nc = a.ncomp - c0 < kCT ? a.ncomp - c0 : kCT;
-        // Per-tile phasors (start, step) of this stage's components.
+        // Per-tile phasors (start, step) of this stage's components, phases reduced in FP64.
         for (int e = tid; e < kCT * 32; e += 256) {
             const int slot = e & 31, cc = e >> 5;
-            double4 ph = make_double4(1.0, 0.0, 1.0, 0.0);
+            double ps = 0.0, pc = 1.0, qs = 0.0, qc = 1.0;
             if (cc < nc) {
                 const double4 l = lmn4[c0 + cc];
                 const double th = s_uvw[slot][0] * l.x + s_uvw[slot][1] * l.y + s_uvw[slot][2] * l.z;
-                fast_sincos(th * f0c, ph.y, ph.x);
-                fast_sincos(th * dfc, ph.w, ph.z);
+                fast_sincos(th * f0c, ps, pc);
+                fast_sincos(th * dfc, qs, qc);
             }
+            V4 ph;
+            ph.x = (T)pc;
+            ph.y = (T)ps;
+            ph.z = (T)qc;
+            ph.w = (T)qs;
             s_ph[cc][slot] = ph;
         }
         __syncthreads();
         for (int cc = 0; cc < nc; ++cc) {
             const int c = c0 + cc;
-            const double4 A = s_ph[cc][sp];
-            const double4 B = s_ph[cc][sq];
-            double zr = A.x * B.x + A.y * B.y;
-            double zi = A.y * B.x - A.x * B.y;
-            const double rr = A.z * B.z + A.w * B.w;
-            const double ri = A.w * B.z - A.z * B.w;
-            double env = 1.0, er = 1.0, eg = 1.0;
+            const V4 A = s_ph[cc][sp];
+            const V4 B = s_ph[cc][sq];
+            T zr = A.x * B.x + A.y * B.y;
+            T zi = A.y * B.x - A.x * B.y;
+            const T rr = A.z * B.z + A.w * B.w;
+            const T ri = A.w * B.z - A.z * B.w;
+            T env = T(1), er = T(1), eg = T(1);
             if (KIND == KIND_GAUSSIAN) {
                 const double4 g = shape[c];
                 const double qf = g.x * (bu * bu) + g.y * (bu * bv) + g.z * (bv * bv);
-                env = exp(qf * (F0 * F0));
-                er = exp(qf * (DF * (2.0 * F0 + DF)));
-                eg = exp(qf * (2.0 * DF * DF));
+                exp_of(qf * (F0 * F0), env);
+                exp_of(qf * (DF * (2.0 * F0 + DF)), er);
+                exp_of(qf * (2.0 * DF * DF), eg);
             }
-            const double *cc8 = coh + ((size_t)c * a.nchan_pad + f0) * 8;
+            const T *cc8 = coh + ((size_t)c * a.nchan_pad + f0) * 8;
 #pragma unroll
             for (int k = 0; k < NCH; ++k) {
-                double pr_ = zr, pi_ = zi;
+                T pr_ = zr, pi_ = zi;
                 if (KIND == KIND_GAUSSIAN) {
                     pr_ *= env;
                     pi_ *= env;
                 }
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    const double cr = cc8[k * 8 + 2 * e], ci = cc8[k * 8 + 2 * e + 1];
-                    acc[k][2 * e] = fma(cr, pr_, acc[k][2 * e]);
-                    acc[k][2 * e] = fma(-ci, pi_, acc[k][2 * e]);
-                    acc[k][2 * e + 1] = fma(cr, pi_, acc[k][2 * e + 1]);
-                    acc[k][2 * e + 1] = fma(ci, pr_, acc[k][2 * e + 1]);
+                    const T cr = cc8[k * 8 + 2 * e], ci = cc8[k * 8 + 2 * e + 1];
+                    acc[k][2 * e] = fma_t(cr, pr_, acc[k][2 * e]);
+                    acc[k][2 * e] = fma_t(-ci, pi_, acc[k][2 * e]);
+                    acc[k][2 * e + 1] = fma_t(cr, pi_, acc[k][2 * e + 1]);
+                    acc[k][2 * e + 1] = fma_t(ci, pr_, acc[k][2 * e + 1]);
                 }
-                const double nr = zr * rr - zi * ri;
+                const T nr = zr * rr - zi * ri;
                 zi = zr * ri + zi * rr;
                 zr = nr;
                 if (KIND == KIND_GAUSSIAN) {
@@ -167,7 +224,7 @@ __global__ __launch_bounds__(256) void k_direct_tiled(const TiledArgs a) {
 #pragma unroll
     for (int k = 0; k < NCH; ++k) {
         const int f = f0 + k;
-        if (f >= a.nchan) break;
+        if (k >= cnt) break;
         const size_t o = (size_t)f * a.nbl + bl;
         float *v32 = reinterpret_cast<float *>(&a.vis32[o]);
 #pragma unroll
@@ -175,7 +232,7 @@ __global__ __launch_bounds__(256) void k_direct_tiled(const TiledArgs a) {
         if (a.vis64 != nullptr) {
             double *v64 = reinterpret_cast<double *>(&a.vis64[o]);
 #pragma unroll
-            for (int e = 0; e < 8; ++e) v64[e] += acc[k][e];
+            for (int e = 0; e < 8; ++e) v64[e] += (double)acc[k][e];
         }
     }
 }
@@ -196,12 +253,22 @@ hipError_t launch_direct_tiled(Kind kind, const TiledArgs &a_in, hipStream_t str
     if (a_in.ncomp <= 0 || a_in.nbl <= 0 || a_in.nchan <= 0) return hipSuccess;
     TiledArgs a = a_in;
     const int nch = kind == KIND_GAUSSIAN ? kTiledNchGaussian : kTiledNchPoint;
-    a.nchunks = (a.nchan + nch - 1) / nch;
+    a.nchunks = chunk_count(a.chunks, nch, a.nchan);
     const int total = a.npairs * a.nchunks;
-    if (kind == KIND_GAUSSIAN)
-        hipLaunchKernelGGL((k_direct_tiled<KIND_GAUSSIAN, kTiledNchGaussian>), dim3(total), dim3(256), 0, stream, a);
-    else
-        hipLaunchKernelGGL((k_direct_tiled<KIND_POINT, kTiledNchPoint>), dim3(total), dim3(256), 0, stream, a);
+    if (a.coh32 != nullptr) {
+        if (kind == KIND_GAUSSIAN)
+            hipLaunchKernelGGL((k_direct_tiled<KIND_GAUSSIAN, kTiledNchGaussian, float>), dim3(total), dim3(256), 0,
+                               stream, a);
+        else
+            hipLaunchKernelGGL((k_direct_tiled<KIND_POINT, kTiledNchPoint, float>), dim3(total), dim3(256), 0, stream,
+                               a);
+    } else if (kind == KIND_GAUSSIAN) {
+        hipLaunchKernelGGL((k_direct_tiled<KIND_GAUSSIAN, kTiledNchGaussian, double>), dim3(total), dim3(256), 0,
+                           stream, a);
+    } else {
+        hipLaunchKernelGGL((k_direct_tiled<KIND_POINT, kTiledNchPoint, double>), dim3(total), dim3(256), 0, stream,
+                           a);
+    }
     return hipGetLastError();
 }
 
@@ -218,13 +285,16 @@ constexpr int kRT = 4 + 64;  // tile slots per block: 4 rows, then 64 columns
 // kernel: on the 256-tile, 384-channel case 236 -> 227 ms per timestep.
 constexpr int kRectNchGaussian = 4;
 
-template <int KIND, int NCH>
+template <int KIND, int NCH, class T>
 __global__ __launch_bounds__(256) void k_direct_rect(const RectArgs a) {
-    __shared__ double4 s_ph[kRC][kRT];
+    using V4 = typename Vec4<T>::type;
+    __shared__ V4 s_ph[kRC][kRT];
     __shared__ double s_uvw[kRT][3];
     const int nrg = (a.n_row + 3) / 4, ncg = (a.n_col + 63) / 64;
     const int b = blockIdx.x;
     const int chunk = b / (nrg * ncg);
+    int f0, cnt;
+    chunk_range<NCH>(a.chunks, chunk, a.nchan, f0, cnt);
     const int rem = b - chunk * (nrg * ncg);
     const int rg = rem / ncg, cg = rem - rg * ncg;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
@@ -250,70 +320,74 @@ __global__ __launch_bounds__(256) void k_direct_rect(const RectArgs a) {
     const double bu = s_uvw[sp][0] - s_uvw[sq][0];
     const double bv = s_uvw[sp][1] - s_uvw[sq][1];
 
-    const int f0 = chunk * NCH;
     const double *__restrict__ freqs = a.freqs;
     const double F0 = freqs[f0];
     const double DF = freqs[f0 + 1] - F0;
     const double f0c = F0 / SM_VEL_C;
     const double dfc = DF / SM_VEL_C;
-    const double *__restrict__ coh = a.coh_cls + (size_t)cls * a.ncomp * a.nchan_pad * 8;
+    const T *__restrict__ coh = pick<T>(a.coh_cls, a.coh_cls32) + (size_t)cls * a.ncomp * a.nchan_pad * 8;
     const double4 *__restrict__ lmn4 = a.lmn;
     const double4 *__restrict__ shape = a.shape;
 
-    double acc[NCH][8];
+    T acc[NCH][8];
 #pragma unroll
     for (int k = 0; k < NCH; ++k)
 #pragma unroll
-        for (int e = 0; e < 8; ++e) acc[k][e] = 0.0;
+        for (int e = 0; e < 8; ++e) acc[k][e] = T(0);
 
     for (int c0 = 0; c0 < a.ncomp; c0 += kRC) {
         const int nc = a.ncomp - c0 < kRC ? a.ncomp - c0 : kRC;
         for (int e = tid; e < kRC * kRT; e += 256) {
             const int cc = e / kRT, slot = e - cc * kRT;
-            double4 ph = make_double4(1.0, 0.0, 1.0, 0.0);
+            double ps = 0.0, pc = 1.0, qs = 0.0, qc = 1.0;
             if (cc < nc) {
                 const double4 l = lmn4[c0 + cc];
                 const double th = s_uvw[slot][0] * l.x + s_uvw[slot][1] * l.y + s_uvw[slot][2] * l.z;
-                fast_sincos(th * f0c, ph.y, ph.x);
-                fast_sincos(th * dfc, ph.w, ph.z);
+                fast_sincos(th * f0c, ps, pc);
+                fast_sincos(th * dfc, qs, qc);
             }
+            V4 ph;
+            ph.x = (T)pc;
+            ph.y = (T)ps;
+            ph.z = (T)qc;
+            ph.w = (T)qs;
             s_ph[cc][slot] = ph;
         }
         __syncthreads();
         if (row_ok) {
             for (int cc = 0; cc < nc; ++cc) {
                 const int c = c0 + cc;
-                const double4 A = s_ph[cc][sp];
-                const double4 B = s_ph[cc][sq];
-                double zr = A.x * B.x + A.y * B.y;
-                double zi = A.y * B.x - A.x * B.y;
-                const double rr = A.z * B.z + A.w * B.w;
-                const double ri_ = A.w * B.z - A.z * B.w;
-                double env = 1.0, er = 1.0, eg = 1.0;
+                const V4 A = s_ph[cc][sp];
+                const V4 B = s_ph[cc][sq];
+                T zr = A.x * B.x + A.y * B.y;
+                T zi = A.y * B.x - A.x * B.y;
+                const T rr = A.z * B.z + A.w * B.w;
+                const T ri_ = A.w * B.z - A.z * B.w;
+                T env = T(1), er = T(1), eg = T(1);
                 if (KIND == KIND_GAUSSIAN) {
                     const double4 g = shape[c];
                     const double qf = g.x * (bu * bu) + g.y * (bu * bv) + g.z * (bv * bv);
-                    env = exp(qf * (F0 * F0));
-                    er = exp(qf * (DF * (2.0 * F0 + DF)));
-                    eg = exp(qf * (2.0 * DF * DF));
+                    exp_of(qf * (F0 * F0), env);
+                    exp_of(qf * (DF * (2.0 * F0 + DF)), er);
+                    exp_of(qf * (2.0 * DF * DF), eg);
                 }
-                const double *cc8 = coh + ((size_t)c * a.nchan_pad + f0) * 8;
+                const T *cc8 = coh + ((size_t)c * a.nchan_pad + f0) * 8;
 #pragma unroll
                 for (int k = 0; k < NCH; ++k) {
-                    double pr_ = zr, pi_ = zi;
+                    T pr_ = zr, pi_ = zi;
                     if (KIND == KIND_GAUSSIAN) {
                         pr_ *= env;
                         pi_ *= env;
                     }
 #pragma unroll
                     for (int e = 0; e < 4; ++e) {
-                        const double cr = cc8[k * 8 + 2 * e], ci = cc8[k * 8 + 2 * e + 1];
-                        acc[k][2 * e] = fma(cr, pr_, acc[k][2 * e]);
-                        acc[k][2 * e] = fma(-ci, pi_, acc[k][2 * e]);
-                        acc[k][2 * e + 1] = fma(cr, pi_, acc[k][2 * e + 1]);
-                        acc[k][2 * e + 1] = fma(ci, pr_, acc[k][2 * e + 1]);
+                        const T cr = cc8[k * 8 + 2 * e], ci = cc8[k * 8 + 2 * e + 1];
+                        acc[k][2 * e] = fma_t(cr, pr_, acc[k][2 * e]);
+                        acc[k][2 * e] = fma_t(-ci, pi_, acc[k][2 * e]);
+                        acc[k][2 * e + 1] = fma_t(cr, pi_, acc[k][2 * e + 1]);
+                        acc[k][2 * e + 1] = fma_t(ci, pr_, acc[k][2 * e + 1]);
                     }
-                    const double nr = zr * rr - zi * ri_;
+                    const T nr = zr * rr - zi * ri_;
                     zi = zr * ri_ + zi * rr;
                     zr = nr;
                     if (KIND == KIND_GAUSSIAN) {
@@ -334,8 +408,8 @@ __global__ __launch_bounds__(256) void k_direct_rect(const RectArgs a) {
 #pragma unroll
     for (int k = 0; k < NCH; ++k) {
         const int f = f0 + k;
-        if (f >= a.nchan) break;
-        double v[8];
+        if (k >= cnt) break;
+        T v[8];
         if (flip) {
             // (V^H)_ij = conj(V_ji): XX, YX, XY, YY with the imaginary parts negated.
             v[0] = acc[k][0];
@@ -357,7 +431,7 @@ __global__ __launch_bounds__(256) void k_direct_rect(const RectArgs a) {
         if (a.vis64 != nullptr) {
             double *v64 = reinterpret_cast<double *>(&a.vis64[o]);
 #pragma unroll
-            for (int e = 0; e < 8; ++e) v64[e] += v[e];
+            for (int e = 0; e < 8; ++e) v64[e] += (double)v[e];
         }
     }
 }
@@ -365,18 +439,26 @@ __global__ __launch_bounds__(256) void k_direct_rect(const RectArgs a) {
 hipError_t launch_direct_rect(Kind kind, const RectArgs &a, hipStream_t stream) {
     if (a.ncomp <= 0 || a.n_row <= 0 || a.n_col <= 0 || a.nchan <= 0) return hipSuccess;
     const int nch = kind == KIND_GAUSSIAN ? kRectNchGaussian : kTiledNchPoint;
-    const long total = (long)((a.n_row + 3) / 4) * ((a.n_col + 63) / 64) * ((a.nchan + nch - 1) / nch);
-    if (kind == KIND_GAUSSIAN)
-        hipLaunchKernelGGL((k_direct_rect<KIND_GAUSSIAN, kRectNchGaussian>), dim3((unsigned)total), dim3(256), 0,
+    const long total = (long)((a.n_row + 3) / 4) * ((a.n_col + 63) / 64) * chunk_count(a.chunks, nch, a.nchan);
+    if (a.coh_cls32 != nullptr) {
+        if (kind == KIND_GAUSSIAN)
+            hipLaunchKernelGGL((k_direct_rect<KIND_GAUSSIAN, kRectNchGaussian, float>), dim3((unsigned)total),
+                               dim3(256), 0, stream, a);
+        else
+            hipLaunchKernelGGL((k_direct_rect<KIND_POINT, kTiledNchPoint, float>), dim3((unsigned)total), dim3(256), 0,
+                               stream, a);
+    } else if (kind == KIND_GAUSSIAN) {
+        hipLaunchKernelGGL((k_direct_rect<KIND_GAUSSIAN, kRectNchGaussian, double>), dim3((unsigned)total),
+                           dim3(256), 0, stream, a);
+    } else {
+        hipLaunchKernelGGL((k_direct_rect<KIND_POINT, kTiledNchPoint, double>), dim3((unsigned)total), dim3(256), 0,
                            stream, a);
-    else
-        hipLaunchKernelGGL((k_direct_rect<KIND_POINT, kTiledNchPoint>), dim3((unsigned)total), dim3(256), 0, stream,
-                           a);
+    }
     return hipGetLastError();
 }
 
 // ------------------------------------------------------------------ shapelets
-template <bool PER_TILE>
+template <bool PER_TILE, class T>
 __device__ __forceinline__ void shapelet_chan(const ShapeletChanArgs &a) {
     const int nbx = (a.nbl + 3) / 4;
     const int total = nbx * a.ngroups;
@@ -410,9 +492,10 @@ __device__ __forceinline__ void shapelet_chan(const ShapeletChanArgs &a) {
         }
     }
     const PackedCoeff *__restrict__ packed = a.packed;
-    const double *__restrict__ sbf = a.sbf;
+    const T *__restrict__ sbf = pick<T>(a.sbf, a.sbf32);
+    const T *__restrict__ coh_t = pick<T>(a.coh, a.coh32);
     // Per-wave windows of the basis table (kRows rows x kWin cells per axis).
-    __shared__ double wx_lds[4][kRows][kWin], wy_lds[4][kRows][kWin];
+    __shared__ T wx_lds[4][kRows][kWin], wy_lds[4][kRows][kWin];
     const int wv = threadIdx.x >> 6;
     // The wave's lowest and highest channel frequency (invalid lanes repeat the last channel).
     double fmin_w = freq, fmax_w = freq;
@@ -420,7 +503,7 @@ __device__ __forceinline__ void shapelet_chan(const ShapeletChanArgs &a) {
         fmin_w = fmin(fmin_w, __shfl_xor(fmin_w, o));
         fmax_w = fmax(fmax_w, __shfl_xor(fmax_w, o));
     }
-    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    T acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     // The wave's baseline is uniform; the next component's parameters are loaded one
     // iteration ahead.
     const int blu = __builtin_amdgcn_readfirstlane(bl);
@@ -439,8 +522,8 @@ __device__ __forceinline__ void shapelet_chan(const ShapeletChanArgs &a) {
             pr_next = a.prange[cn];
         }
         const double theta = uvw.u * l.x + uvw.v * l.y + uvw.w * l.z;
-        double zr, zi;
-        fast_sincos(theta * fc, zi, zr);
+        T zr, zi;
+        sincos_phase(theta * fc, zi, zr);
         const double xk = (suv.x * sh.x + suv.y * sh.y) * sh.z;
         const double yk = (suv.x * sh.y - suv.y * sh.x) * sh.w;
         const double x_pos = xk * freq + SM_SBF_C;
@@ -448,8 +531,8 @@ __device__ __forceinline__ void shapelet_chan(const ShapeletChanArgs &a) {
         const double fx = floor(x_pos), fy = floor(y_pos);
         const int ix = (int)fmin(fmax(fx, 0.0), (double)(SM_SBF_L - 2));
         const int iy = (int)fmin(fmax(fy, 0.0), (double)(SM_SBF_L - 2));
-        const double wx = x_pos - fx, wy = y_pos - fy;
-        double er, ei;
+        const T wx = (T)(x_pos - fx), wy = (T)(y_pos - fy);
+        T er, ei;
         // The cells the wave touches: positions are monotone in frequency, so the extreme
         // channels bound them (the same expressions, so the same roundings).
         const double xa = xk * fmin_w + SM_SBF_C, xb = xk * fmax_w + SM_SBF_C;
@@ -460,8 +543,8 @@ __device__ __forceinline__ void shapelet_chan(const ShapeletChanArgs &a) {
         const int iy1 = (int)fmin(fmax(floor(fmax(ya, yb)), 0.0), (double)(SM_SBF_L - 2));
         const int xw = __builtin_amdgcn_readfirstlane(ix1 - ix0 + 2);
         const int yw = __builtin_amdgcn_readfirstlane(iy1 - iy0 + 2);
-        double cpre[8];
-        const double *s_c = a.coh + ((size_t)c * a.nchan_pad + f) * 8;
+        T cpre[8];
+        const T *s_c = coh_t + ((size_t)c * a.nchan_pad + f) * 8;
         if (xw <= kWin && yw <= kWin && pr4.w <= kRows) {
             // Stage rows 0 .. pr4.w - 1 of both windows in this wave's LDS, then interpolate
             // from LDS: one 16-byte read per axis per coefficient instead of two gathers.
@@ -473,18 +556,18 @@ __device__ __forceinline__ void shapelet_chan(const ShapeletChanArgs &a) {
             const int j = lane & (kWin - 1), half = lane / kWin;
             const int jxc = min(j, xw - 1), jyc = min(j, yw - 1);
             static_assert(kRows <= SM_SBF_N, "staged rows must exist in the basis table");
-            const unsigned ox = (unsigned)((half * SM_SBF_L + bx0 + jxc) * 8);
-            const unsigned oy = (unsigned)((half * SM_SBF_L + by0 + jyc) * 8);
+            const unsigned ox = (unsigned)((half * SM_SBF_L + bx0 + jxc) * sizeof(T));
+            const unsigned oy = (unsigned)((half * SM_SBF_L + by0 + jyc) * sizeof(T));
             const char *sbf_b = reinterpret_cast<const char *>(sbf);
             constexpr int kBatch = 2;  // row steps in flight at once (register budget)
             for (int i0 = 0; i0 * (64 / kWin) < pr4.w; i0 += kBatch) {
-                double tx[kBatch], ty[kBatch];
+                T tx[kBatch], ty[kBatch];
 #pragma unroll
                 for (int i = 0; i < kBatch; ++i) {
-                    const unsigned step = (unsigned)((i0 + i) * (64 / kWin) * SM_SBF_L * 8);
+                    const unsigned step = (unsigned)((i0 + i) * (64 / kWin) * SM_SBF_L * sizeof(T));
                     if ((i0 + i) * (64 / kWin) < pr4.w) {
-                        tx[i] = *reinterpret_cast<const double *>(sbf_b + (ox + step));
-                        ty[i] = *reinterpret_cast<const double *>(sbf_b + (oy + step));
+                        tx[i] = *reinterpret_cast<const T *>(sbf_b + (ox + step));
+                        ty[i] = *reinterpret_cast<const T *>(sbf_b + (oy + step));
                     }
                 }
 #pragma unroll
@@ -504,22 +587,22 @@ __device__ __forceinline__ void shapelet_chan(const ShapeletChanArgs &a) {
 #pragma unroll
                 for (int e = 0; e < 8; ++e) cpre[e] = s_c[e];
             }
-            double part[2];
+            T part[2];
 #pragma unroll
             for (int pt = 0; pt < 2; ++pt) {
                 const PackedCoeff *cp = packed + pr4.x + (pt ? pr4.y : 0);
                 const int cnt = pt ? pr4.z : pr4.y;
-                double accp = 0.0;
+                T accp = T(0);
 #pragma unroll 2
                 for (int k = 0; k < cnt; ++k) {
                     double val;
                     int n1, n2;
                     load_packed_uniform(cp + k, val, n1, n2);
-                    const double xl = wx_lds[wv][n1][jx], xh = wx_lds[wv][n1][jx + 1];
-                    const double yl = wy_lds[wv][n2][jy], yh = wy_lds[wv][n2][jy + 1];
-                    const double u_value = xl + (xh - xl) * wx;
-                    const double v_value = yl + (yh - yl) * wy;
-                    accp += (val * u_value) * v_value;
+                    const T xl = wx_lds[wv][n1][jx], xh = wx_lds[wv][n1][jx + 1];
+                    const T yl = wy_lds[wv][n2][jy], yh = wy_lds[wv][n2][jy + 1];
+                    const T u_value = xl + (xh - xl) * wx;
+                    const T v_value = yl + (yh - yl) * wy;
+                    accp += ((T)val * u_value) * v_value;
                 }
                 part[pt] = accp;
             }
@@ -532,21 +615,21 @@ __device__ __forceinline__ void shapelet_chan(const ShapeletChanArgs &a) {
         } else {
             // Rare (a window wider than the LDS copy): one coefficient at a time, which keeps
             // this path's registers below the staged path's.
-            double part[2];
+            T part[2];
 #pragma unroll
             for (int pt = 0; pt < 2; ++pt) {
                 const PackedCoeff *cp = packed + pr4.x + (pt ? pr4.y : 0);
                 const int cnt = pt ? pr4.z : pr4.y;
-                double accp = 0.0;
+                T accp = T(0);
 #pragma unroll 1
                 for (int k = 0; k < cnt; ++k) {
                     double val;
                     int n1, n2;
                     load_packed_uniform(cp + k, val, n1, n2);
-                    const double *bx = sbf + n1 * SM_SBF_L + ix, *by = sbf + n2 * SM_SBF_L + iy;
-                    const double u_value = bx[0] + (bx[1] - bx[0]) * wx;
-                    const double v_value = by[0] + (by[1] - by[0]) * wy;
-                    accp += (val * u_value) * v_value;
+                    const T *bx = sbf + n1 * SM_SBF_L + ix, *by = sbf + n2 * SM_SBF_L + iy;
+                    const T u_value = bx[0] + (bx[1] - bx[0]) * wx;
+                    const T v_value = by[0] + (by[1] - by[0]) * wy;
+                    accp += ((T)val * u_value) * v_value;
                 }
                 part[pt] = accp;
             }
@@ -557,29 +640,33 @@ __device__ __forceinline__ void shapelet_chan(const ShapeletChanArgs &a) {
                 for (int e = 0; e < 8; ++e) cpre[e] = s_c[e];
             }
         }
-        const double pr = zr * er - zi * ei;
-        const double pi = zr * ei + zi * er;
-        double cj[8];
-        const double *s = a.coh + ((size_t)c * a.nchan_pad + f) * 8;
+        const T pr = zr * er - zi * ei;
+        const T pi = zr * ei + zi * er;
+        T cj[8];
         if (PER_TILE && cls >= 0) {
-            class_coh(a.beam.coh_cls + (((size_t)cls * a.ncomp + c) * a.nchan_pad + f) * 8, herm, cj);
+            class_coh(pick<T>(a.beam.coh_cls, a.coh_cls32) + (((size_t)cls * a.ncomp + c) * a.nchan_pad + f) * 8,
+                      herm, cj);
         } else if (PER_TILE) {
-            double sv[8], j1[8], j2[8];
+            // J_p S J_q^H of the instrumental flux S (FP64 table) and the two tiles' beams.
+            const double *s = a.coh + ((size_t)c * a.nchan_pad + f) * 8;
+            double sv[8], j1[8], j2[8], cd[8];
 #pragma unroll
             for (int e = 0; e < 8; ++e) sv[e] = s[e];
             load_jones_t(&a.beam.jones[((size_t)rows.x * a.beam.n_ufreq + uf) * a.beam.n_dir + a.beam.dir_offset + c], j1);
             load_jones_t(&a.beam.jones[((size_t)rows.y * a.beam.n_ufreq + uf) * a.beam.n_dir + a.beam.dir_offset + c], j2);
-            sandwich_t(j1, sv, j2, cj);
+            sandwich_t(j1, sv, j2, cd);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) cj[e] = (T)cd[e];
         } else {
 #pragma unroll
             for (int e = 0; e < 8; ++e) cj[e] = cpre[e];
         }
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            acc[2 * e] = fma(cj[2 * e], pr, acc[2 * e]);
-            acc[2 * e] = fma(-cj[2 * e + 1], pi, acc[2 * e]);
-            acc[2 * e + 1] = fma(cj[2 * e], pi, acc[2 * e + 1]);
-            acc[2 * e + 1] = fma(cj[2 * e + 1], pr, acc[2 * e + 1]);
+            acc[2 * e] = fma_t(cj[2 * e], pr, acc[2 * e]);
+            acc[2 * e] = fma_t(-cj[2 * e + 1], pi, acc[2 * e]);
+            acc[2 * e + 1] = fma_t(cj[2 * e], pi, acc[2 * e + 1]);
+            acc[2 * e + 1] = fma_t(cj[2 * e + 1], pr, acc[2 * e + 1]);
         }
     }
     if (!valid) return;
@@ -590,11 +677,12 @@ __device__ __forceinline__ void shapelet_chan(const ShapeletChanArgs &a) {
     if (a.vis64 != nullptr) {
         double *v64 = reinterpret_cast<double *>(&a.vis64[o]);
 #pragma unroll
-        for (int e = 0; e < 8; ++e) v64[e] += acc[e];
+        for (int e = 0; e < 8; ++e) v64[e] += (double)acc[e];
     }
 }
 
-__global__ __launch_bounds__(256) void k_shapelet_chan(const ShapeletChanArgs a) { shapelet_chan<false>(a); }
+__global__ __launch_bounds__(256) void k_shapelet_chan(const ShapeletChanArgs a) { shapelet_chan<false, double>(a); }
+__global__ __launch_bounds__(256) void k_shapelet_chan32(const ShapeletChanArgs a) { shapelet_chan<false, float>(a); }
 
 // Per-tile beams: held to 128 VGPRs (4 waves per SIMD, a few spilled values) instead of the
 // 164 the compiler picks (3 waves). Measured on the 256-tile, 384-channel case: 129 -> 111 ms
@@ -602,16 +690,33 @@ __global__ __launch_bounds__(256) void k_shapelet_chan(const ShapeletChanArgs a)
 // 3.7% slower.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_shapelet_chan_tiles(
     const ShapeletChanArgs a) {
-    shapelet_chan<true>(a);
+    shapelet_chan<true, double>(a);
+}
+__global__ __launch_bounds__(256) void k_shapelet_chan_tiles32(const ShapeletChanArgs a) {
+    shapelet_chan<true, float>(a);
 }
 
 hipError_t launch_shapelet_chan(bool per_tile, const ShapeletChanArgs &a, hipStream_t stream) {
     if (a.ncomp <= 0 || a.nbl <= 0 || a.nchan <= 0) return hipSuccess;
     const int total = ((a.nbl + 3) / 4) * a.ngroups;
+    const bool f32 = a.coh32 != nullptr;
+    if (f32 && (a.sbf32 == nullptr || (per_tile && a.beam.coh_cls != nullptr && a.coh_cls32 == nullptr)))
+        return hipErrorInvalidValue;
     if (per_tile)
-        hipLaunchKernelGGL(k_shapelet_chan_tiles, dim3(total), dim3(256), 0, stream, a);
+        hipLaunchKernelGGL(f32 ? k_shapelet_chan_tiles32 : k_shapelet_chan_tiles, dim3(total), dim3(256), 0, stream, a);
     else
-        hipLaunchKernelGGL(k_shapelet_chan, dim3(total), dim3(256), 0, stream, a);
+        hipLaunchKernelGGL(f32 ? k_shapelet_chan32 : k_shapelet_chan, dim3(total), dim3(256), 0, stream, a);
+    return hipGetLastError();
+}
+
+__global__ void k_to_f32(size_t n, const double *__restrict__ in, float *__restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (float)in[i];
+}
+
+hipError_t launch_to_f32(size_t n, const double *d_in, float *d_out, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, n, d_in, d_out);
     return hipGetLastError();
 }
 

@@ -210,6 +210,9 @@ struct SmModeller {
     bool even = false;
     // device
     DevBuf d_freqs, d_xyz, d_bl_tiles, d_bl_rows, d_tile_rows, d_freq_map, d_ufreqs, d_delays, d_gains, d_sbf;
+    DevBuf d_sbf32;  // the basis table in FP32 (precision path)
+    DevBuf d_chunks4, d_chunks8;  // evenly spaced channel runs of at most 4 and 8 channels
+    int nchunks4 = 0, nchunks8 = 0;
     DevBuf d_uvw, d_vis32, d_vis64, d_accum, d_tile_uvw, d_pairs;
     // split-precision point path
     DevBuf d_fblocks, d_fblock_of, d_fout, d_fscratch, d_fkbuf, d_fcohf, d_fmap;
@@ -551,6 +554,16 @@ extern "C" const char *sm_new(const SmObsParams *p, const SmComponent *comps, co
     SM_NEW_TRY(upload(m->d_gains, m->gains));
     SM_NEW_TRY(upload(m->d_sbf, shapelet_table()));
     {
+        const std::vector<int2> c4 = even_chunks(m->freqs_pad.data(), m->nchan, 4);
+        const std::vector<int2> c8 = even_chunks(m->freqs_pad.data(), m->nchan, 8);
+        m->nchunks4 = (int)c4.size();
+        m->nchunks8 = (int)c8.size();
+        SM_NEW_TRY(upload(m->d_chunks4, c4));
+        SM_NEW_TRY(upload(m->d_chunks8, c8));
+    }
+    SM_NEW_TRY(m->d_sbf32.reserve(sizeof(float) * (size_t)SM_SBF_N * SM_SBF_L));
+    SM_NEW_TRY(launch_to_f32((size_t)SM_SBF_N * SM_SBF_L, m->d_sbf.as<double>(), m->d_sbf32.as<float>(), 0));
+    {
         const std::vector<int4> pairs = tile_pairs(m->nt);
         m->npairs = (int)pairs.size();
         SM_NEW_TRY(upload(m->d_pairs, pairs));
@@ -749,12 +762,20 @@ static const char *model_timestep_device(SmModeller *m, double lst, double latit
     ctx.nchan_pad = m->nchan_pad;
     ctx.ntiles = m->nt;
     ctx.d_freqs_pad = m->d_freqs.as<double>();
-    ctx.even = m->even;
+    // Chunks are runs of evenly spaced channels: every kernel takes its evenly spaced form
+    // whatever channels are flagged.
+    ctx.chunks.c4 = m->d_chunks4.as<int2>();
+    ctx.chunks.n4 = m->nchunks4;
+    ctx.chunks.c8 = m->d_chunks8.as<int2>();
+    ctx.chunks.n8 = m->nchunks8;
+    ctx.even = true;
     ctx.d_uvw = uvw;
     ctx.d_sbf = m->d_sbf.as<double>();
     ctx.d_tile_uvw = tile_uvw;
     ctx.d_pairs = m->d_pairs.as<int4>();
     ctx.npairs = m->npairs;
+    ctx.fp32 = factored_dtype >= 0;
+    ctx.d_sbf32 = m->d_sbf32.as<float>();
     if (m->g0_row >= 0) {
         ctx.nclass = m->g0_nclass;
         ctx.class_row0 = m->g0_row;

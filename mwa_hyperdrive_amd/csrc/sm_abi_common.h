@@ -21,7 +21,8 @@ struct TypeInputs {
 struct ModelCtx {
     int nbl = 0, nchan = 0, nchan_pad = 0, ntiles = 0;
     const double *d_freqs_pad = nullptr;
-    bool even = false;
+    bool even = false;   // every chunk evenly spaced (always, with chunk tables)
+    ChunkTabs chunks;    // channel chunks of evenly spaced runs, or null (fixed chunks)
     const UVW *d_uvw = nullptr;
     const double *d_sbf = nullptr;
     BeamDesc beam{};
@@ -54,6 +55,11 @@ struct ModelCtx {
     const int *h_class_row = nullptr;
     const int *d_class_row = nullptr;   // [nclass] beam row of each class
     const int *d_class_of_row = nullptr;  // [n_utile]
+    // Precision path: Gaussians and shapelets on the tile-pair, row-by-column and
+    // channel-per-lane kernels run in FP32 with FP64 phases (d_sbf32 = the basis table in
+    // FP32). Points, the per-tile baseline kernel and the per-segment fallback stay FP64.
+    bool fp32 = false;
+    const float *d_sbf32 = nullptr;
 };
 
 // 0: pick the fastest kernel for the case; 1: always the baseline-per-lane kernels.
@@ -69,6 +75,8 @@ struct Scratch {
     DevBuf coh0;     // J_0 S J_0^H of beam-row class 0
     DevBuf cseg;     // one segment's shared coherency (fallback of the row-by-column kernel)
     DevBuf ccls;     // row-by-column kernel: J_r S J_0^H per beam-row class
+    // FP32 copies for the FP32-arithmetic kernels (ModelCtx::fp32)
+    DevBuf coh32, coh0_32, ccls32, cls32;
 };
 
 // Largest per-class coherency table (bytes) run_type builds; above it, or when the

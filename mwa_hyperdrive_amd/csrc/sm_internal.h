@@ -17,6 +17,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "../../include/skymodel.h"
 
 #define SM_NCH 8
@@ -56,6 +58,43 @@ struct BeamDesc {
     int row0 = -1;
 };
 
+// Channel chunks of the direct kernels. With tables: runs of at most 4 (c4) or 8 (c8)
+// evenly spaced channels, (first channel, count), so that a flagged channel only shortens
+// the runs around it; every chunk is evenly spaced. Null tables: fixed chunks of 4 or 8
+// consecutive channels, evenly spaced or not.
+struct ChunkTabs {
+    const int2 *c4 = nullptr;
+    int n4 = 0;
+    const int2 *c8 = nullptr;
+    int n8 = 0;
+};
+
+// Runs of at most `width` channels f0, f0 + 1, ... with freqs[f0 + k] == freqs[f0] + k (freqs[f0 + 1]
+// - freqs[f0]) exactly, covering all nchan channels in order.
+std::vector<int2> even_chunks(const double *freqs, int nchan, int width);
+
+// Number of chunks of width nch (4 or 8).
+__host__ __device__ inline int chunk_count(const ChunkTabs &t, int nch, int nchan) {
+    const int2 *tab = nch == 4 ? t.c4 : t.c8;
+    if (tab != nullptr) return nch == 4 ? t.n4 : t.n8;
+    return (nchan + nch - 1) / nch;
+}
+
+// First channel and channel count of chunk `chunk` of width NCH.
+template <int NCH>
+__device__ __forceinline__ void chunk_range(const ChunkTabs &t, int chunk, int nchan, int &f0, int &cnt) {
+    static_assert(NCH == 4 || NCH == 8, "chunk tables exist for widths 4 and 8");
+    const int2 *tab = NCH == 4 ? t.c4 : t.c8;
+    if (tab != nullptr) {
+        const int2 c = tab[chunk];
+        f0 = c.x;
+        cnt = c.y;
+    } else {
+        f0 = chunk * NCH;
+        cnt = nchan - f0 < NCH ? nchan - f0 : NCH;
+    }
+}
+
 // A shapelet coefficient as the kernels read it: the value with the sign of
 // i^((n1 + n2) mod 4) folded in, and the basis-table rows n1, n2. Per component the terms
 // that land in the real part of the envelope come first, then those that land in the
@@ -86,6 +125,7 @@ struct DirectArgs {
     // FP64, set up by launch_direct) before one pass adds them to the visibilities.
     int csplit = 1;
     double *accum = nullptr;
+    ChunkTabs chunks;  // channel chunks (evenly spaced runs), or fixed chunks when null
 };
 
 // Launchers (direct.hip). All enqueue on `stream` and return hipSuccess or an error.

@@ -80,8 +80,8 @@ __device__ __forceinline__ void load_jones(const JonesF64 *p, double j[8]) {
 
 // A class coherency J_r S J_0^H, or its conjugate transpose J_0 S J_r^H (S is Hermitian)
 // when herm: entry (i, j) from (j, i), conjugated.
-__device__ __forceinline__ void class_coh(const double *p, bool herm, double o[8]) {
-    double v[8];
+template <class T> __device__ __forceinline__ void class_coh(const T *p, bool herm, T o[8]) {
+    T v[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) v[e] = p[e];
     o[0] = v[0];

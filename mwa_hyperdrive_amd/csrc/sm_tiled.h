@@ -16,12 +16,16 @@ struct TiledArgs {
     const double4 *lmn;
     const double4 *shape;
     const double *coh;        // [ncomp][nchan_pad][8]
+    // FP32 arithmetic (phases and their range reduction still FP64): the coherency table in
+    // FP32, [ncomp][nchan_pad][8]. Null selects the FP64 kernels.
+    const float *coh32 = nullptr;
     JonesF32 *vis32;
     JonesF64 *vis64;
     // Optional tile subset: kernel tile i is array tile tile_map[i] (ascending), and output
     // baselines are numbered over ntiles_bl array tiles.
     const int *tile_map = nullptr;
     int ntiles_bl = 0;
+    ChunkTabs chunks;  // channel chunks; the fixed chunks must then be evenly spaced
 };
 
 struct ShapeletChanArgs {
@@ -38,6 +42,11 @@ struct ShapeletChanArgs {
     BeamDesc beam;
     JonesF32 *vis32;
     JonesF64 *vis64;
+    // FP32 arithmetic with FP64 phases and table positions: coherency, per-class coherency
+    // and basis table in FP32. Null coh32 selects the FP64 kernels.
+    const float *coh32 = nullptr;
+    const float *coh_cls32 = nullptr;  // [nclass][ncomp][nchan_pad][8] when beam.coh_cls is set
+    const float *sbf32 = nullptr;      // [SM_SBF_N][SM_SBF_L]
 };
 
 // Pairs of a tile outside beam class 0 (rows) with a class-0 tile (columns): one wave per
@@ -53,8 +62,10 @@ struct RectArgs {
     const double4 *lmn;
     const double4 *shape;
     const double *coh_cls;    // [nclass][ncomp][nchan_pad][8], J_r S J_0^H
+    const float *coh_cls32 = nullptr;  // the same in FP32 (FP32 arithmetic); null: FP64 kernels
     JonesF32 *vis32;
     JonesF64 *vis64;
+    ChunkTabs chunks;  // channel chunks; the fixed chunks must then be evenly spaced
 };
 
 std::vector<int4> tile_pairs(int ntiles);
@@ -62,5 +73,7 @@ hipError_t launch_direct_rect(Kind kind, const RectArgs &a, hipStream_t stream);
 hipError_t launch_direct_tiled(Kind kind, const TiledArgs &a, hipStream_t stream);
 hipError_t launch_shapelet_chan(bool per_tile, const ShapeletChanArgs &a, hipStream_t stream);
 hipError_t launch_tile_uvw_from_baselines(int ntiles, const UVW *d_uvw, double *d_out, hipStream_t stream);
+// FP64 -> FP32 copy of n values (coherency tables, basis table).
+hipError_t launch_to_f32(size_t n, const double *d_in, float *d_out, hipStream_t stream);
 
 }  // namespace sm

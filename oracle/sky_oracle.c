@@ -420,6 +420,7 @@ typedef struct {
     double *uvws;          /* [num_bl][3] */
     int32_t *bl_t1, *bl_t2; /* unflagged-tile indices per baseline */
     type_list types[3];
+    int32_t f_start, f_count; /* the channels being modelled */
 } modeller;
 
 static int is_flagged(const OrcObs *obs, int32_t t) {
@@ -434,6 +435,8 @@ static void build_modeller(modeller *m, const OrcObs *obs, const OrcComponent *c
                            int32_t num_sources, int32_t f_start, int32_t f_count) {
     memset(m, 0, sizeof *m);
     m->obs = obs;
+    m->f_start = f_start;
+    m->f_count = f_count;
     int32_t nt = obs->num_unflagged_tiles;
     m->total_tiles = nt + obs->num_flagged_tiles;
     m->num_bl = nt * (nt - 1) / 2;
@@ -558,11 +561,16 @@ static jones *beam_responses(const modeller *m, const type_list *tl) {
         az[i] = a;
         za[i] = FRAC_PI_2 - e;
     }
+    /* Only the unique frequencies of the channels being modelled are evaluated (the others
+     * are never read). */
+    char *used = (char *)calloc((size_t)m->n_unique_freqs + 1, 1);
+    for (int32_t f = m->f_start; f < m->f_start + m->f_count; f++) used[m->freq_map[f]] = 1;
     for (int32_t ut = 0; ut < m->n_unique_tiles; ut++) {
         int32_t t = m->unique_tiles[ut];
         const uint32_t *dl = obs->dipole_delays ? &obs->dipole_delays[16 * t] : NULL;
         const double *gn = obs->dipole_gains ? &obs->dipole_gains[32 * t] : NULL;
         for (int32_t uf = 0; uf < m->n_unique_freqs; uf++) {
+            if (!used[uf]) continue;
             for (size_t i = 0; i < n; i++) {
                 double jj[8];
                 orc_analytic_beam_jones(az[i], za[i], m->unique_freqs[uf], dl, gn, jj);
@@ -574,6 +582,7 @@ static jones *beam_responses(const modeller *m, const type_list *tl) {
             }
         }
     }
+    free(used);
     free(az);
     free(za);
     return out;

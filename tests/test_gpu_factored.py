@@ -14,7 +14,7 @@ torch = pytest.importorskip("torch")
 
 from mwa_hyperdrive_amd import _lib, synthetic
 from mwa_hyperdrive_amd.modeller import AnalyticBeam, NoBeam, SkyModeller
-from tests.helpers import oracle_model, rel_err
+from tests.helpers import gpu_model, oracle_model, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -146,3 +146,32 @@ def test_factored_kernel_organisations_agree(n_comps):
     finally:
         _lib.call("sm_set_factored_segment", 64)
         _lib.call("sm_set_factored_kernel", 1)
+
+
+@pytest.mark.parametrize("beam", ["shared", "per-tile"])
+def test_fp32_extended_components_vs_oracle(beam):
+    """Gaussians and shapelets on the precision path run in FP32 with FP64 phases (tile-pair,
+    row-by-column and channel-per-lane kernels). Bar: 1e-6 of max|V| against the FP64 oracle;
+    the result must differ from the FP64 path's (it is the FP32 arithmetic that ran) by no
+    more than that."""
+    from mwa_hyperdrive_amd.srclist import Gaussian, Shapelet, SourceList
+    base = synthetic.shrink(synthetic.config(5, scale=0.004), tiles_n=80, chans=37, timesteps=1)
+    sky = SourceList([(k, s) for k, s in base.srclist.items()
+                      if isinstance(s.components[0].comp_type, (Gaussian, Shapelet))])
+    assert len(sky) > 20
+    base.srclist = sky
+    if beam == "shared":
+        base.delays = np.zeros((80, 16), np.uint32)
+        base.gains = np.ones((80, 32))
+    o32, o64 = oracle_model(base, 0)
+    v32, v64, m = factored(base, 0, 0)
+    info = m.info()
+    assert info["num_points"] == 0 and info["num_gaussians"] > 0 and info["num_shapelets"] > 0
+    if beam == "per-tile":
+        assert info["num_unique_tiles"] > 2
+    f64 = gpu_model(base, 0)[1]
+    err = rel_err(v64, o64)
+    diff = rel_err(v64, f64)
+    print(f"{beam} beam: FP32 Gaussians + shapelets rel err vs oracle {err:.3e}, vs FP64 path {diff:.3e}")
+    assert err <= 1e-6
+    assert diff > 1e-12

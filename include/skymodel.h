@@ -185,6 +185,28 @@ const char *sm_set_class_table_limit(int64_t bytes);
 const char *sm_get_flux_params(SmModeller *m, int32_t comp_type, int32_t flux_type, int32_t index, int32_t chan,
                                JonesF64 *fd, double *si, double *q);
 
+/* ------------------------------------------------------------------ peel-loop neighbours
+ * The device work around the modeller in the reference's peel loop
+ * (src/params/peel/gpu.rs:441-470). Same signatures as src/gpu/peel.h:13-35 with
+ * FLOAT = double; device pointers; visibilities [time][freq][baseline] JonesF32, UVWs
+ * [time][baseline], XYZs [time][tile]. Work is enqueued on the null stream. */
+
+/* Replaces xyzs_to_uvws (src/gpu/peel.h:13-14): baseline UVWs (metres) of the tile XYZs at
+ * each timestep's LMST for the pointing centre. */
+const char *xyzs_to_uvws(const XYZ *d_xyzs, const double *d_lmsts, UVW *d_uvws, RADec pointing_centre, int num_tiles,
+                         int num_baselines, int num_timesteps);
+
+/* Replaces rotate (src/gpu/peel.h:16-17): vis *= exp(-i 2 pi (w_to - w_from) / lambda), formed
+ * in FP64 and rounded to FP32. */
+const char *rotate(JonesF32 *d_vis_fb, const int num_timesteps, const int num_baselines, const int num_freqs,
+                   const UVW *d_uvws_from, const UVW *d_uvws_to, const double *d_lambdas);
+
+/* Replaces add_model (src/gpu/peel.h:33-35): residual += model * gain * exp(-i 2 pi (u alpha +
+ * v beta) lambda) (no phase when alpha = beta = 0). */
+const char *add_model(JonesF32 *d_vis_residual, const JonesF32 *d_vis_model, const IonoConsts iono_consts,
+                      const double *d_lambdas_m, const UVW *d_uvws, const int num_timesteps, const int num_baselines,
+                      const int num_freqs);
+
 /* ------------------------------------------------------------------ beam */
 /* Analytic MWA-like beam on the device: out[tile][freq][dir] JonesF64 for the given unique
  * tiles (delays [n_tiles][16], gains [n_tiles][32], host), frequencies (host, Hz) and

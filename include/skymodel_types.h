@@ -38,6 +38,9 @@ typedef struct ShapeletCoeff {
     unsigned char n2;
 } ShapeletCoeff;
 typedef struct ShapeletUV { double u, v; } ShapeletUV; /* metres */
+/* Ionospheric offset constants of the peel loop (src/gpu/types.h:280-287): the phase
+ * offset per metre of u and v, and the gain. */
+typedef struct IonoConsts { double alpha, beta, gain; } IonoConsts;
 
 /* 2x2 complex matrices, [XX, XY, YX, YY] with (re, im) interleaved; X = east-west. */
 typedef struct JonesF32 {

@@ -83,6 +83,14 @@ class Shapelets(C.Structure):
     ]
 
 
+class RADec(C.Structure):
+    _fields_ = [("ra", C.c_double), ("dec", C.c_double)]
+
+
+class IonoConsts(C.Structure):
+    _fields_ = [("alpha", C.c_double), ("beta", C.c_double), ("gain", C.c_double)]
+
+
 class SmObsParams(C.Structure):
     _fields_ = [
         ("num_unflagged_tiles", C.c_int32),
@@ -143,6 +151,9 @@ SIGNATURES = {
     "sm_synchronize": [],
     "sm_set_direct_variant": [_I],
     "sm_set_class_table_limit": [C.c_int64],
+    "xyzs_to_uvws": [_P, _P, _P, RADec, C.c_int, C.c_int, C.c_int],
+    "rotate": [_P, C.c_int, C.c_int, C.c_int, _P, _P, _P],
+    "add_model": [_P, _P, IonoConsts, _P, _P, C.c_int, C.c_int, C.c_int],
     "sm_get_flux_params": [_P, _I, _I, _I, _I, _P, C.POINTER(_D), C.POINTER(_D)],
     "sm_analytic_beam": [_I, _P, _P, _I, _P, _I, _P, _P, _P],
     "sm_model_timestep_factored": [_P, _D, _D, _I, _P, _P],

@@ -19,7 +19,7 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libskymodel_hip.so"
-SOURCES = ["direct.hip", "direct_tiled.hip", "abi.hip", "modeller.hip", "factored.hip", "surrogate.hip"]
+SOURCES = ["direct.hip", "direct_tiled.hip", "abi.hip", "modeller.hip", "factored.hip", "surrogate.hip", "peel.hip"]
 ARCH = os.environ.get("SM_OFFLOAD_ARCH", "gfx950")
 
 

@@ -1,0 +1,132 @@
+"""The peel loop's device neighbours of the modeller (xyzs_to_uvws, rotate, add_model) and the
+per-source device loop itself (src/params/peel/gpu.rs:441-470): update_with_a_source, the
+device-buffer model with UVWs supplied by the caller, and add_model into residuals."""
+
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from mwa_hyperdrive_amd import _lib, synthetic
+from mwa_hyperdrive_amd.modeller import AnalyticBeam, SkyModeller
+from mwa_hyperdrive_amd.srclist import SourceList
+from oracle import oracle as orc
+from oracle import peel_oracle as po
+from tests.helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a, dtype=torch.float64):
+    return torch.tensor(np.ascontiguousarray(a), dtype=dtype, device="cuda")
+
+
+def ptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+@pytest.mark.parametrize("ntiles", [9, 70, 256])
+def test_xyzs_to_uvws_vs_oracle(ntiles):
+    rng = np.random.default_rng(ntiles)
+    xyz = rng.uniform(-3000, 3000, size=(3, ntiles, 3))
+    lmsts = np.array([0.1, 1.7, 4.2])
+    nbl = ntiles * (ntiles - 1) // 2
+    out = torch.zeros((3, nbl, 3), dtype=torch.float64, device="cuda")
+    _lib.call("xyzs_to_uvws", ptr(dev(xyz)), ptr(dev(lmsts)), ptr(out), _lib.RADec(0.3, -0.6), ntiles, nbl, 3)
+    torch.cuda.synchronize()
+    want = po.xyzs_to_uvws(xyz, lmsts, 0.3, -0.6)
+    got = out.cpu().numpy()
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
+
+
+def test_rotate_vs_oracle():
+    rng = np.random.default_rng(8)
+    nt, nf, nb = 3, 7, 300
+    vis = rng.normal(size=(nt, nf, nb, 8)).astype(np.float32)
+    u_from = rng.normal(size=(nt, nb, 3)) * 500
+    u_to = u_from + rng.normal(size=(nt, nb, 3)) * 40
+    lambdas = 299792458.0 / (167e6 + 40e3 * np.arange(nf))
+    d = dev(vis, torch.float32)
+    _lib.call("rotate", ptr(d), nt, nb, nf, ptr(dev(u_from)), ptr(dev(u_to)), ptr(dev(lambdas)))
+    torch.cuda.synchronize()
+    want = po.rotate(vis, u_from, u_to, lambdas)
+    got = d.cpu().numpy()
+    # FP64 products rounded once to FP32: the two sides differ at most by the last bit where
+    # the device and host sin/cos differ in their last FP64 bit.
+    np.testing.assert_allclose(got, want, rtol=2.0 ** -23, atol=1e-30)
+
+
+def test_add_model_reference_kat_and_oracle():
+    """src/params/peel/tests.rs:2524-2600 on the device, then against the restatement."""
+    nt, nc, ntiles = 10, 20, 30
+    nb = ntiles * (ntiles - 1) // 2
+    lambdas = 1.0 + np.arange(nc, dtype=np.float64)
+    model = np.zeros((nt, nc, nb, 8), np.float32)
+    model[...] = np.array([0, 0, 0, 1, 1, 0, 1, 1], np.float32)
+    uvws = np.zeros((nt, nb, 3))
+    uvws[..., 1] = 1.0
+    res = torch.zeros(model.shape, dtype=torch.float32, device="cuda")
+    _lib.call("add_model", ptr(res), ptr(dev(model, torch.float32)), _lib.IonoConsts(0.1, 0.2, 0.3),
+              ptr(dev(lambdas)), ptr(dev(uvws)), nt, nb, nc)
+    torch.cuda.synchronize()
+    got = res.cpu().numpy()
+    norms = np.hypot(got[..., 0::2], got[..., 1::2])
+    want_norm = np.array([0.0, 1.0, 1.0, math.sqrt(2.0)]) * np.float32(0.3)
+    np.testing.assert_allclose(norms, np.broadcast_to(want_norm, norms.shape), rtol=0, atol=1e-7)
+    rng = np.random.default_rng(4)
+    model = rng.normal(size=(2, 5, 40, 8)).astype(np.float32)
+    resid = rng.normal(size=(2, 5, 40, 8)).astype(np.float32)
+    uvws = rng.normal(size=(2, 40, 3)) * 300
+    lam = 299792458.0 / (170e6 + 80e3 * np.arange(5))
+    d = dev(resid, torch.float32)
+    _lib.call("add_model", ptr(d), ptr(dev(model, torch.float32)), _lib.IonoConsts(2e-4, -1e-4, 0.8),
+              ptr(dev(lam)), ptr(dev(uvws)), 2, 40, 5)
+    torch.cuda.synchronize()
+    want = po.add_model(resid, model, 2e-4, -1e-4, 0.8, lam, uvws)
+    # FP32 products and sums; the device may fuse a multiply into the add (one rounding less).
+    assert np.abs(d.cpu().numpy() - want).max() <= 4 * 2.0 ** -24 * (np.abs(resid).max() + np.abs(model).max())
+
+
+def test_peel_source_loop_vs_oracle():
+    """Per source: the modeller takes the source alone with the source as phase centre, the
+    baseline UVWs at the source come from xyzs_to_uvws and are handed to the device-buffer
+    model entry, and add_model adds the ionospherically offset model to the residuals
+    (src/params/peel/gpu.rs:441-470). The oracle does the same with its own UVWs."""
+    base = synthetic.shrink(synthetic.config(2, scale=40 / 8000.0), tiles_n=24, chans=13, timesteps=3)
+    sources = list(base.srclist.items())[:5]
+    nt, nf, ntl = len(base.lsts), len(base.freqs), base.num_tiles
+    nb = base.num_baselines
+    lambdas = 299792458.0 / base.freqs
+    m = SkyModeller(AnalyticBeam(base.delays, base.gains), base.srclist, 0, base.xyzs, base.freqs, [],
+                    base.phase_centre, 0.0, base.latitude)
+    d_xyz = dev(np.broadcast_to(base.xyzs, (nt, ntl, 3)))
+    d_lmst = dev(base.lsts)
+    d_lam = dev(lambdas)
+    resid = torch.zeros((nt, nf, nb, 8), dtype=torch.float32, device="cuda")
+    want = np.zeros((nt, nf, nb, 8), np.float32)
+    iono = [(1e-4 * k, -5e-5 * k, 1.0 - 0.05 * k) for k in range(len(sources))]
+    for k, (name, src) in enumerate(sources):
+        pos = (src.components[0].ra, src.components[0].dec)
+        m.update_with_a_source(src, pos)
+        uvw_to = torch.zeros((nt, nb, 3), dtype=torch.float64, device="cuda")
+        _lib.call("xyzs_to_uvws", ptr(d_xyz), ptr(d_lmst), ptr(uvw_to), _lib.RADec(*pos), ntl, nb, nt)
+        model = torch.zeros((nt, nf, nb, 8), dtype=torch.float32, device="cuda")
+        for t in range(nt):
+            m.model_timestep_with_device(float(base.lsts[t]), base.latitude, model[t], None, uvw_to[t])
+        _lib.call("add_model", ptr(resid), ptr(model), _lib.IonoConsts(*iono[k]), ptr(d_lam), ptr(uvw_to), nt, nb,
+                  nf)
+        o_model = np.zeros((nt, nf, nb, 8), np.float32)
+        o_uvw = np.zeros((nt, nb, 3))
+        for t in range(nt):
+            o = orc.Obs(base.xyzs, base.freqs, pos, float(base.lsts[t]), base.latitude, beam="analytic",
+                        delays=base.delays, gains=base.gains)
+            o_model[t] = orc.model_timestep(o, SourceList([(name, src)]))[0]
+            o_uvw[t] = orc.cross_uvws(base.xyzs, float(base.lsts[t]) - pos[0], pos[1])
+        want = po.add_model(want, o_model, *iono[k], lambdas, o_uvw)
+    torch.cuda.synchronize()
+    err = rel_err(resid.cpu().numpy(), want)
+    print(f"peel loop over {len(sources)} sources: rel err {err:.3e}")
+    assert err <= 1e-6

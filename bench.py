@@ -36,11 +36,13 @@ sys.path.insert(0, str(ROOT))
 METRIC = "Gcomp·vis/s, FP64 direct vs split-precision MFMA at ≤1e-6 rel err vs FP64"
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
-# Algorithmic FP64 work of the point direct kernel per comp.vis (sincos counted as one op
-# each): phase u L + v M + w N (5) + sin, cos (2) + coherency x phasor (24) + accumulate (8).
-# The J S J^H product (112 FLOP) is computed once per (component, channel) by the
-# coherency kernel, not per baseline.
-FLOP_PER_COMPVIS_POINT = 39.0
+# Algorithmic FP64 work of the point tile-pair kernel per comp.vis: 4 complex multiply-adds of
+# coherency and phasor (32), the phasor's advance across the chunk's channels (6), the
+# tile-phasor product at the chunk's first channel shared by its 4 channels (1.5), and the
+# tile phasors themselves, shared by 256 baselines (under 0.5). The reference kernel's per
+# comp.vis J S J^H (112 FLOP) and per-baseline sincos are not part of this algorithm: J S J^H
+# is formed once per (component, channel) and the phase factorises over antennas.
+FLOP_PER_COMPVIS_POINT = 40.0
 
 
 def cpu_model_name() -> str:
@@ -203,8 +205,34 @@ def split_precision() -> dict:
         res[name] = entry
         print(f"[bench] split precision {name}: {entry['gcompvis_per_s_kernel']:.1f} Gcomp.vis/s (kernels)",
               file=sys.stderr, flush=True)
+    # The same fp16 path with the K operand read from pre-split planes (factored kernel 2).
+    try:
+        _lib.call("sm_set_factored_kernel", 2)
+        run(0, float(obs.lsts[0]))
+        torch.cuda.synchronize()
+        m.set_timing(True)
+        t0 = time.perf_counter()
+        for lst in obs.lsts:
+            run(0, float(lst))
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        info = m.info()
+        m.set_timing(False)
+        kern_ms = info["kernel_ms_factored"]
+        d64 = torch.zeros(shape, dtype=torch.float64, device="cuda")
+        run(0, float(obs.lsts[5]), d64)
+        torch.cuda.synchronize()
+        res["fp16_hi_lo_planes"] = {
+            "kernel_ms": kern_ms, "wall_ms": wall * 1e3, "gcompvis_per_s_kernel": work / (kern_ms * 1e-3) / 1e9,
+            "gcompvis_per_s_wall": work / wall / 1e9,
+            "rel_err_vs_fp64": (d64 - ref).reshape(-1, 2).norm(dim=1).max().item() / scale}
+        del d64
+        print(f"[bench] split precision fp16 (K planes): {res['fp16_hi_lo_planes']['gcompvis_per_s_kernel']:.1f} "
+              "Gcomp.vis/s (kernels)", file=sys.stderr, flush=True)
+    finally:
+        _lib.call("sm_set_factored_kernel", 1)
     base = res["fp64_direct"]["gcompvis_per_s_kernel"]
-    for name in ("fp16_hi_lo", "bf16_hi_lo"):
+    for name in ("fp16_hi_lo", "bf16_hi_lo", "fp16_hi_lo_planes"):
         res[name]["speedup_vs_fp64_direct"] = res[name]["gcompvis_per_s_kernel"] / base
     # MFMA rate of the fp16 path, executed dense work: per channel and component, each 64 x 64
     # tile block computes 4 Jones entries x 64 x 128 outputs = 32 output tiles of 32 x 32
@@ -222,9 +250,10 @@ def split_precision() -> dict:
 
 def combined_c5() -> dict:
     """C5 (256 tiles, per-tile beams with dead dipoles, 50 000 points + 10 000 Gaussians + 500
-    shapelets, 384 channels): one timestep through the FP64 direct path and through the
-    split-precision path (fp16 points on MFMA by beam-row classes), after one warm-up call
-    each; error of the latter against the former."""
+    shapelets, 384 channels, 28 timesteps): every timestep through the FP64 direct path and
+    through the precision path (fp16 points on MFMA by beam-row classes, Gaussians and shapelets
+    in FP32 with FP64 phases), after one warm-up call each; the error of the latter against the
+    former at one timestep."""
     import ctypes as C
 
     import torch
@@ -240,36 +269,65 @@ def combined_c5() -> dict:
     ref = torch.zeros(shape, dtype=torch.float64, device="cuda")
     out = torch.zeros(shape, dtype=torch.float64, device="cuda")
     lst = float(obs.lsts[3])
+    nts = len(obs.lsts)
     work = obs.num_components * obs.num_baselines * len(obs.freqs)
 
-    def fp64(d64):
-        m.model_timestep_with_device(lst, obs.latitude, v32, d64)
+    def fp64(d64, t_lst=None):
+        m.model_timestep_with_device(lst if t_lst is None else t_lst, obs.latitude, v32, d64)
 
-    def mixed(d64):
-        _lib.call("sm_model_timestep_factored", m._h, lst, obs.latitude, 0, C.c_void_p(v32.data_ptr()),
-                  C.c_void_p(d64.data_ptr()))
+    def mixed(d64, t_lst=None):
+        _lib.call("sm_model_timestep_factored", m._h, lst if t_lst is None else t_lst, obs.latitude, 0,
+                  C.c_void_p(v32.data_ptr()), C.c_void_p(d64.data_ptr()) if d64 is not None else None)
 
     res = {"workload": f"C5: {obs.num_baselines} baselines, {len(obs.freqs)} channels, "
                        f"{obs.num_components} components, per-tile beams ({m.info()['num_unique_tiles']} beam rows), "
-                       "one timestep", "comp_vis": work}
+                       f"{nts} timesteps", "comp_vis": work * nts}
     for name, fn, buf in (("fp64_direct", fp64, ref), ("fp16_points_mfma", mixed, out)):
-        fn(buf)
+        fn(buf)  # warm-up; this call's output is the error reference
         torch.cuda.synchronize()
-        buf.zero_()
         m.set_timing(True)
         t0 = time.perf_counter()
-        fn(buf)
+        for t in range(nts):
+            fn(None, float(obs.lsts[t]))
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         info = m.info()
         m.set_timing(False)
-        res[name] = {"wall_ms": wall * 1e3, "gcompvis_per_s_wall": work / wall / 1e9,
-                     "kernel_ms": {k: info[f"kernel_ms_{k}"] for k in ("points", "factored", "gaussians", "shapelets")}}
-        print(f"[bench] C5 {name}: {wall * 1e3:.1f} ms", file=sys.stderr, flush=True)
+        res[name] = {"wall_ms": wall * 1e3, "ms_per_timestep": wall * 1e3 / nts,
+                     "gcompvis_per_s_wall": work * nts / wall / 1e9,
+                     "kernel_ms_per_timestep": {k: info[f"kernel_ms_{k}"] / nts
+                                                for k in ("points", "factored", "gaussians", "shapelets")}}
+        print(f"[bench] C5 {name}: {wall * 1e3 / nts:.1f} ms per timestep", file=sys.stderr, flush=True)
     scale = ref.reshape(-1, 2).norm(dim=1).max().item()
     res["fp16_points_mfma"]["rel_err_vs_fp64"] = (out - ref).reshape(-1, 2).norm(dim=1).max().item() / scale
     res["speedup_end_to_end"] = res["fp64_direct"]["wall_ms"] / res["fp16_points_mfma"]["wall_ms"]
-    del m, v32, ref, out
+    # The same precision path with the beam from the MLP surrogate and the exact beam as per-tile
+    # fallback above a 1e-3 Jones-error threshold (a short training run first).
+    from mwa_hyperdrive_amd import surrogate as sg
+
+    s = sg.BeamSurrogate(seed=1)
+    tr = s.train(n_samples=1 << 21, epochs=4, batch=4096, lr=2e-3)
+    m.set_beam_surrogate(s, 1e-3)
+    out.zero_()
+    mixed(out)
+    torch.cuda.synchronize()
+    m.set_beam_surrogate(s, 1e-3)
+    m.set_timing(True)
+    t0 = time.perf_counter()
+    for t in range(nts):
+        mixed(None, float(obs.lsts[t]))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    m.set_timing(False)
+    fallbacks = m.set_beam_surrogate(None, 0.0)
+    res["fp16_points_mfma_surrogate_beam"] = {
+        "wall_ms": wall * 1e3, "ms_per_timestep": wall * 1e3 / nts, "gcompvis_per_s_wall": work * nts / wall / 1e9,
+        "threshold": 1e-3, "tile_fallbacks": fallbacks,
+        "tile_beams_evaluated": 3 * nts * m.info()["num_unique_tiles"], "train_seconds": tr["seconds"],
+        "rel_err_vs_fp64": (out - ref).reshape(-1, 2).norm(dim=1).max().item() / scale}
+    print(f"[bench] C5 surrogate beam: {wall * 1e3 / nts:.1f} ms per timestep, {fallbacks} tile fallbacks",
+          file=sys.stderr, flush=True)
+    del m, v32, ref, out, s
     torch.cuda.empty_cache()
     return res
 
@@ -349,8 +407,8 @@ def main() -> int:
     ap.add_argument("--no-split-precision", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--probe", action="store_true",
-                    help="counter runs only: one plain 1 GiB device copy (the calibration kernel) and one C2 "
-                         "timestep, no JSON")
+                    help="counter runs only: one plain 1 GiB device copy (the calibration kernel), one C2 "
+                         "timestep (FP64) and one C3 timestep (fp16 split precision), no JSON")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -376,6 +434,18 @@ def main() -> int:
                         obs.phase_centre, 0.0, obs.latitude)
         vis = torch.zeros((len(obs.freqs), obs.num_baselines, 8), dtype=torch.float32, device="cuda")
         m.model_timestep_with_device(float(obs.lsts[0]), obs.latitude, vis)
+        torch.cuda.synchronize()
+        del m, vis
+        # One C3 timestep through the fp16 split-precision MFMA path.
+        import ctypes as C
+
+        from mwa_hyperdrive_amd import _lib
+        from mwa_hyperdrive_amd.modeller import NoBeam
+        c3 = synthetic.config(3)
+        m3 = SkyModeller(NoBeam(), c3.srclist, 0, c3.xyzs, c3.freqs, [], c3.phase_centre, 0.0, c3.latitude)
+        v3 = torch.zeros((len(c3.freqs), c3.num_baselines, 8), dtype=torch.float32, device="cuda")
+        _lib.call("sm_model_timestep_factored", m3._h, float(c3.lsts[0]), c3.latitude, 0, C.c_void_p(v3.data_ptr()),
+                  None)
         torch.cuda.synchronize()
         print("[bench] probe done", file=sys.stderr, flush=True)
         return 0

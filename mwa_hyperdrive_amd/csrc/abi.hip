@@ -2,7 +2,6 @@
 // shared per-component-type driver used by both these and the SkyModeller handle.
 #include <cmath>
 #include <cstring>
-#include <mutex>
 #include <vector>
 
 #include "sm_abi_common.h"
@@ -281,9 +280,32 @@ hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf
             u.nsub = cnt;
             e = launch_subset(kind, false, ctx.even, u, scratch.accum, stream);
         }
-        // Pairs of two tiles outside class 0: per-tile J_p S J_q^H.
+        // Pairs of two tiles outside class 0: per-tile J_p S J_q^H, on the own-beam tile-pair
+        // kernel (both tiles' Jones staged in LDS), else on the baseline kernel's subset form.
         const int off = ctx.h_seg_off[nseg], cnt = ctx.h_seg_off[nseg + 1] - off;
-        if (e == hipSuccess && cnt > 0) {
+        const bool pairs_ok = ctx.n_other >= 2 && ctx.d_other_tiles != nullptr && ctx.beam.tile_rows != nullptr &&
+                              (long)cnt == (long)ctx.n_other * (ctx.n_other - 1) / 2;
+        if (e == hipSuccess && cnt > 0 && pairs_ok) {
+            PairJonesArgs pj;
+            pj.ntiles_list = ctx.n_other;
+            pj.ntiles_bl = ctx.ntiles;
+            pj.nbl = ctx.nbl;
+            pj.nchan = ctx.nchan;
+            pj.nchan_pad = ctx.nchan_pad;
+            pj.ncomp = n;
+            pj.tiles = ctx.d_other_tiles;
+            pj.tile_uvw = ctx.d_tile_uvw;
+            pj.freqs = ctx.d_freqs_pad;
+            pj.lmn = in.lmn;
+            pj.shape = in.shape;
+            pj.coh = coh.as<double>();
+            pj.beam = ctx.beam;
+            pj.fp32 = f32;
+            pj.vis32 = vis32;
+            pj.vis64 = vis64;
+            pj.chunks = ctx.chunks;
+            e = launch_pair_jones(kind, pj, stream);
+        } else if (e == hipSuccess && cnt > 0) {
             DirectArgs u = a;
             u.bl_subset = ctx.d_notg0_bl + off;
             u.nsub = cnt;

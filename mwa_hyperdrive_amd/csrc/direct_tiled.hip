@@ -80,6 +80,26 @@ __device__ __forceinline__ void sincos_phase(double x, float &s, float &c) {
     c = ((q + 1) & 2) ? -b : b;
 }
 
+// FP32 kernels sum in two levels: a stage's components (at most 32) add into the inner
+// accumulators, which are then added into the outer ones, so the rounding of a sum over N
+// components grows like sqrt(N / stage) rather than sqrt(N). FP64 kernels sum in one level.
+template <class T> struct TwoLevel { static constexpr bool on = false; };
+template <> struct TwoLevel<float> { static constexpr bool on = true; };
+
+template <class T, int N>
+__device__ __forceinline__ void flush_stage(T (&acc)[N][8], T (&tot)[N][8]) {
+    if (!TwoLevel<T>::on) return;
+#pragma unroll
+    for (int k = 0; k < N; ++k)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            tot[k][e] += acc[k][e];
+            acc[k][e] = T(0);
+        }
+}
+
+template <class T> __device__ __forceinline__ T two_level_sum(T acc, T tot) { return TwoLevel<T>::on ? tot + acc : acc; }
+
 template <class T> __device__ __forceinline__ const T *pick(const double *d, const float *f);
 template <> __device__ __forceinline__ const double *pick<double>(const double *d, const float *) { return d; }
 template <> __device__ __forceinline__ const float *pick<float>(const double *, const float *f) { return f; }
@@ -145,11 +165,11 @@ __global__ __launch_bounds__(256) void k_direct_tiled(const TiledArgs a) {
     const double4 *__restrict__ lmn4 = a.lmn;
     const double4 *__restrict__ shape = a.shape;
 
-    T acc[NCH][8];
+    T acc[NCH][8], tot[NCH][8];
 #pragma unroll
     for (int k = 0; k < NCH; ++k)
 #pragma unroll
-        for (int e = 0; e < 8; ++e) acc[k][e] = T(0);
+        for (int e = 0; e < 8; ++e) acc[k][e] = tot[k][e] = T(0);
 
     for (int c0 = 0; c0 < a.ncomp; c0 += kCT) {
         const int nc = a.ncomp - c0 < kCT ? a.ncomp - c0 : kCT;
@@ -212,6 +232,7 @@ __global__ __launch_bounds__(256) void k_direct_tiled(const TiledArgs a) {
                 }
             }
         }
+        flush_stage(acc, tot);
         __syncthreads();
     }
     if (!valid) return;
@@ -228,11 +249,11 @@ __global__ __launch_bounds__(256) void k_direct_tiled(const TiledArgs a) {
         const size_t o = (size_t)f * a.nbl + bl;
         float *v32 = reinterpret_cast<float *>(&a.vis32[o]);
 #pragma unroll
-        for (int e = 0; e < 8; ++e) v32[e] += (float)acc[k][e];
+        for (int e = 0; e < 8; ++e) v32[e] += (float)two_level_sum(acc[k][e], tot[k][e]);
         if (a.vis64 != nullptr) {
             double *v64 = reinterpret_cast<double *>(&a.vis64[o]);
 #pragma unroll
-            for (int e = 0; e < 8; ++e) v64[e] += (double)acc[k][e];
+            for (int e = 0; e < 8; ++e) v64[e] += (double)two_level_sum(acc[k][e], tot[k][e]);
         }
     }
 }
@@ -252,12 +273,14 @@ std::vector<int4> tile_pairs(int ntiles) {
 hipError_t launch_direct_tiled(Kind kind, const TiledArgs &a_in, hipStream_t stream) {
     if (a_in.ncomp <= 0 || a_in.nbl <= 0 || a_in.nchan <= 0) return hipSuccess;
     TiledArgs a = a_in;
-    const int nch = kind == KIND_GAUSSIAN ? kTiledNchGaussian : kTiledNchPoint;
+    // FP32 Gaussians take 4 channels per lane: their exponentials are cheap hardware exp2
+    // calls, and the two-level accumulators of 8 channels would double the registers.
+    const int nch = kind == KIND_GAUSSIAN && a.coh32 == nullptr ? kTiledNchGaussian : kTiledNchPoint;
     a.nchunks = chunk_count(a.chunks, nch, a.nchan);
     const int total = a.npairs * a.nchunks;
     if (a.coh32 != nullptr) {
         if (kind == KIND_GAUSSIAN)
-            hipLaunchKernelGGL((k_direct_tiled<KIND_GAUSSIAN, kTiledNchGaussian, float>), dim3(total), dim3(256), 0,
+            hipLaunchKernelGGL((k_direct_tiled<KIND_GAUSSIAN, kTiledNchPoint, float>), dim3(total), dim3(256), 0,
                                stream, a);
         else
             hipLaunchKernelGGL((k_direct_tiled<KIND_POINT, kTiledNchPoint, float>), dim3(total), dim3(256), 0, stream,
@@ -268,6 +291,215 @@ hipError_t launch_direct_tiled(Kind kind, const TiledArgs &a_in, hipStream_t str
     } else {
         hipLaunchKernelGGL((k_direct_tiled<KIND_POINT, kTiledNchPoint, double>), dim3(total), dim3(256), 0, stream,
                            a);
+    }
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ pairs with own beams
+constexpr int kPJC = 8;  // components per LDS stage
+constexpr int kPJN = 4;  // channels per lane
+
+// 2x2 complex (A B^H)[i][j] = sum_k A[i][k] conj(B[j][k]) added to acc with phasor z:
+// acc += z (A B^H).
+template <class T>
+__device__ __forceinline__ void acc_abh(const T *a, const T *b, T zr, T zi, T *acc) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            T mr = T(0), mi = T(0);
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const T ar = a[4 * i + 2 * k], ai = a[4 * i + 2 * k + 1];
+                const T br = b[4 * j + 2 * k], bi = b[4 * j + 2 * k + 1];
+                mr = fma_t(ar, br, mr);
+                mr = fma_t(ai, bi, mr);
+                mi = fma_t(ai, br, mi);
+                mi = fma_t(-ar, bi, mi);
+            }
+            const int e = 2 * (2 * i + j);
+            acc[e] = fma_t(mr, zr, acc[e]);
+            acc[e] = fma_t(-mi, zi, acc[e]);
+            acc[e + 1] = fma_t(mr, zi, acc[e + 1]);
+            acc[e + 1] = fma_t(mi, zr, acc[e + 1]);
+        }
+}
+
+template <int KIND, class T>
+__global__ __launch_bounds__(256) void k_pair_jones(const PairJonesArgs a, int ngroups) {
+    using V4 = typename Vec4<T>::type;
+    __shared__ V4 s_ph[kPJC][32];
+    // [component][channel][slot]: slots 0..15 hold J_p S of the row tiles, 16..31 J_q of the
+    // column tiles; a wave's 16 column lanes read 16 consecutive slots.
+    __shared__ T s_m[kPJC][kPJN][32][8];
+    __shared__ double s_uvw[32][3];
+    __shared__ int s_row[32];
+    const int ngpairs = ngroups * (ngroups + 1) / 2;
+    const int chunk = blockIdx.x / ngpairs;
+    // Tile-group pair (g0 <= g1) of this block, in row-major order of the upper triangle.
+    int2 gp = make_int2(0, 0);
+    {
+        int rem = blockIdx.x - chunk * ngpairs, g0 = 0;
+        while (rem >= ngroups - g0) {
+            rem -= ngroups - g0;
+            ++g0;
+        }
+        gp = make_int2(g0, g0 + rem);
+    }
+    int f0, cnt;
+    chunk_range<kPJN>(a.chunks, chunk, a.nchan, f0, cnt);
+    const int tid = threadIdx.x, r = tid >> 4, cq = tid & 15;
+    if (tid < 32) {
+        const int g = tid < 16 ? gp.x : gp.y;
+        const int i = g * 16 + (tid & 15);
+        const int t = a.tiles[i < a.ntiles_list ? i : a.ntiles_list - 1];
+        s_uvw[tid][0] = a.tile_uvw[3 * t];
+        s_uvw[tid][1] = a.tile_uvw[3 * t + 1];
+        s_uvw[tid][2] = a.tile_uvw[3 * t + 2];
+        s_row[tid] = a.beam.tile_rows[t];
+    }
+    const int ip = gp.x * 16 + r, iq = gp.y * 16 + cq;
+    const bool valid = ip < a.ntiles_list && iq < a.ntiles_list && ip < iq;
+    __syncthreads();
+    const double bu = s_uvw[r][0] - s_uvw[16 + cq][0];
+    const double bv = s_uvw[r][1] - s_uvw[16 + cq][1];
+    const double F0 = a.freqs[f0];
+    const double DF = a.freqs[f0 + 1] - F0;
+    const double f0c = F0 / SM_VEL_C, dfc = DF / SM_VEL_C;
+
+    T acc[kPJN][8], tot[kPJN][8];
+#pragma unroll
+    for (int k = 0; k < kPJN; ++k)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) acc[k][e] = tot[k][e] = T(0);
+
+    for (int c0 = 0; c0 < a.ncomp; c0 += kPJC) {
+        const int nc = a.ncomp - c0 < kPJC ? a.ncomp - c0 : kPJC;
+        {
+            // One phasor pair per (component, tile slot): 8 x 32 = 256 threads.
+            const int cc = tid >> 5, slot = tid & 31;
+            double ps = 0.0, pc = 1.0, qs = 0.0, qc = 1.0;
+            if (cc < nc) {
+                const double4 l = a.lmn[c0 + cc];
+                const double th = s_uvw[slot][0] * l.x + s_uvw[slot][1] * l.y + s_uvw[slot][2] * l.z;
+                fast_sincos(th * f0c, ps, pc);
+                fast_sincos(th * dfc, qs, qc);
+            }
+            V4 ph;
+            ph.x = (T)pc;
+            ph.y = (T)ps;
+            ph.z = (T)qc;
+            ph.w = (T)qs;
+            s_ph[cc][slot] = ph;
+        }
+        // J_p S (row slots) and J_q (column slots) per (component, slot, channel).
+        for (int w = tid; w < kPJC * 32 * kPJN; w += 256) {
+            const int k = w % kPJN, slot = (w / kPJN) & 31, cc = w / (kPJN * 32);
+            const int f = f0 + k < a.nchan ? f0 + k : a.nchan - 1;
+            T *o = s_m[cc][k][slot];
+            if (cc >= nc) {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) o[e] = T(0);
+                continue;
+            }
+            const int c = c0 + cc;
+            double j[8];
+            load_jones(&a.beam.jones[((size_t)s_row[slot] * a.beam.n_ufreq + a.beam.freq_map[f]) * a.beam.n_dir +
+                                     a.beam.dir_offset + c],
+                       j);
+            if (slot < 16) {
+                double sv[8];
+                load_jones(reinterpret_cast<const JonesF64 *>(a.coh + ((size_t)c * a.nchan_pad + f) * 8), sv);
+                // (J S)[i][l] = sum_k J[i][k] S[k][l]
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int l = 0; l < 2; ++l) {
+                        double re = 0.0, im = 0.0;
+#pragma unroll
+                        for (int kk = 0; kk < 2; ++kk) {
+                            const double jr = j[4 * i + 2 * kk], ji = j[4 * i + 2 * kk + 1];
+                            const double sr = sv[4 * kk + 2 * l], si = sv[4 * kk + 2 * l + 1];
+                            re += jr * sr - ji * si;
+                            im += jr * si + ji * sr;
+                        }
+                        o[4 * i + 2 * l] = (T)re;
+                        o[4 * i + 2 * l + 1] = (T)im;
+                    }
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) o[e] = (T)j[e];
+            }
+        }
+        __syncthreads();
+        for (int cc = 0; cc < nc; ++cc) {
+            const V4 A = s_ph[cc][r];
+            const V4 B = s_ph[cc][16 + cq];
+            T zr = A.x * B.x + A.y * B.y;
+            T zi = A.y * B.x - A.x * B.y;
+            const T rr = A.z * B.z + A.w * B.w;
+            const T ri = A.w * B.z - A.z * B.w;
+            T env = T(1), er = T(1), eg = T(1);
+            if (KIND == KIND_GAUSSIAN) {
+                const double4 g = a.shape[c0 + cc];
+                const double qf = g.x * (bu * bu) + g.y * (bu * bv) + g.z * (bv * bv);
+                exp_of(qf * (F0 * F0), env);
+                exp_of(qf * (DF * (2.0 * F0 + DF)), er);
+                exp_of(qf * (2.0 * DF * DF), eg);
+            }
+#pragma unroll
+            for (int k = 0; k < kPJN; ++k) {
+                T pr = zr, pi = zi;
+                if (KIND == KIND_GAUSSIAN) {
+                    pr *= env;
+                    pi *= env;
+                }
+                acc_abh<T>(s_m[cc][k][r], s_m[cc][k][16 + cq], pr, pi, acc[k]);
+                const T nr = zr * rr - zi * ri;
+                zi = zr * ri + zi * rr;
+                zr = nr;
+                if (KIND == KIND_GAUSSIAN) {
+                    env *= er;
+                    er *= eg;
+                }
+            }
+        }
+        flush_stage(acc, tot);
+        __syncthreads();
+    }
+    if (!valid) return;
+    const int p = a.tiles[ip], q = a.tiles[iq];
+    const size_t nt = (size_t)a.ntiles_bl;
+    const size_t bl = (size_t)p * nt - (size_t)p * (p + 1) / 2 + (size_t)(q - p - 1);
+#pragma unroll
+    for (int k = 0; k < kPJN; ++k) {
+        if (k >= cnt) break;
+        const size_t o = (size_t)(f0 + k) * a.nbl + bl;
+        float *v32 = reinterpret_cast<float *>(&a.vis32[o]);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v32[e] += (float)two_level_sum(acc[k][e], tot[k][e]);
+        if (a.vis64 != nullptr) {
+            double *v64 = reinterpret_cast<double *>(&a.vis64[o]);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) v64[e] += (double)two_level_sum(acc[k][e], tot[k][e]);
+        }
+    }
+}
+
+hipError_t launch_pair_jones(Kind kind, const PairJonesArgs &a, hipStream_t stream) {
+    if (a.ncomp <= 0 || a.ntiles_list < 2 || a.nchan <= 0) return hipSuccess;
+    if (kind == KIND_SHAPELET) return hipErrorInvalidValue;
+    const int G = (a.ntiles_list + 15) / 16;
+    const long total = (long)(G * (G + 1) / 2) * chunk_count(a.chunks, kPJN, a.nchan);
+    if (a.fp32) {
+        if (kind == KIND_GAUSSIAN)
+            hipLaunchKernelGGL((k_pair_jones<KIND_GAUSSIAN, float>), dim3((unsigned)total), dim3(256), 0, stream, a, G);
+        else
+            hipLaunchKernelGGL((k_pair_jones<KIND_POINT, float>), dim3((unsigned)total), dim3(256), 0, stream, a, G);
+    } else if (kind == KIND_GAUSSIAN) {
+        hipLaunchKernelGGL((k_pair_jones<KIND_GAUSSIAN, double>), dim3((unsigned)total), dim3(256), 0, stream, a, G);
+    } else {
+        hipLaunchKernelGGL((k_pair_jones<KIND_POINT, double>), dim3((unsigned)total), dim3(256), 0, stream, a, G);
     }
     return hipGetLastError();
 }
@@ -329,11 +561,11 @@ __global__ __launch_bounds__(256) void k_direct_rect(const RectArgs a) {
     const double4 *__restrict__ lmn4 = a.lmn;
     const double4 *__restrict__ shape = a.shape;
 
-    T acc[NCH][8];
+    T acc[NCH][8], tot[NCH][8];
 #pragma unroll
     for (int k = 0; k < NCH; ++k)
 #pragma unroll
-        for (int e = 0; e < 8; ++e) acc[k][e] = T(0);
+        for (int e = 0; e < 8; ++e) acc[k][e] = tot[k][e] = T(0);
 
     for (int c0 = 0; c0 < a.ncomp; c0 += kRC) {
         const int nc = a.ncomp - c0 < kRC ? a.ncomp - c0 : kRC;
@@ -397,6 +629,7 @@ __global__ __launch_bounds__(256) void k_direct_rect(const RectArgs a) {
                 }
             }
         }
+        flush_stage(acc, tot);
         __syncthreads();
     }
     if (!valid) return;
@@ -409,20 +642,22 @@ __global__ __launch_bounds__(256) void k_direct_rect(const RectArgs a) {
     for (int k = 0; k < NCH; ++k) {
         const int f = f0 + k;
         if (k >= cnt) break;
-        T v[8];
+        T v[8], w[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) w[e] = two_level_sum(acc[k][e], tot[k][e]);
         if (flip) {
             // (V^H)_ij = conj(V_ji): XX, YX, XY, YY with the imaginary parts negated.
-            v[0] = acc[k][0];
-            v[1] = -acc[k][1];
-            v[2] = acc[k][4];
-            v[3] = -acc[k][5];
-            v[4] = acc[k][2];
-            v[5] = -acc[k][3];
-            v[6] = acc[k][6];
-            v[7] = -acc[k][7];
+            v[0] = w[0];
+            v[1] = -w[1];
+            v[2] = w[4];
+            v[3] = -w[5];
+            v[4] = w[2];
+            v[5] = -w[3];
+            v[6] = w[6];
+            v[7] = -w[7];
         } else {
 #pragma unroll
-            for (int e = 0; e < 8; ++e) v[e] = acc[k][e];
+            for (int e = 0; e < 8; ++e) v[e] = w[e];
         }
         const size_t o = (size_t)f * a.nbl + bl;
         float *v32 = reinterpret_cast<float *>(&a.vis32[o]);
@@ -503,7 +738,7 @@ __device__ __forceinline__ void shapelet_chan(const ShapeletChanArgs &a) {
         fmin_w = fmin(fmin_w, __shfl_xor(fmin_w, o));
         fmax_w = fmax(fmax_w, __shfl_xor(fmax_w, o));
     }
-    T acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    T acc[1][8] = {{0, 0, 0, 0, 0, 0, 0, 0}}, tot[1][8] = {{0, 0, 0, 0, 0, 0, 0, 0}};
     // The wave's baseline is uniform; the next component's parameters are loaded one
     // iteration ahead.
     const int blu = __builtin_amdgcn_readfirstlane(bl);
@@ -663,21 +898,22 @@ __device__ __forceinline__ void shapelet_chan(const ShapeletChanArgs &a) {
         }
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            acc[2 * e] = fma_t(cj[2 * e], pr, acc[2 * e]);
-            acc[2 * e] = fma_t(-cj[2 * e + 1], pi, acc[2 * e]);
-            acc[2 * e + 1] = fma_t(cj[2 * e], pi, acc[2 * e + 1]);
-            acc[2 * e + 1] = fma_t(cj[2 * e + 1], pr, acc[2 * e + 1]);
+            acc[0][2 * e] = fma_t(cj[2 * e], pr, acc[0][2 * e]);
+            acc[0][2 * e] = fma_t(-cj[2 * e + 1], pi, acc[0][2 * e]);
+            acc[0][2 * e + 1] = fma_t(cj[2 * e], pi, acc[0][2 * e + 1]);
+            acc[0][2 * e + 1] = fma_t(cj[2 * e + 1], pr, acc[0][2 * e + 1]);
         }
+        if ((c & 15) == 15) flush_stage(acc, tot);
     }
     if (!valid) return;
     const size_t o = (size_t)f * a.nbl + bl;
     float *v32 = reinterpret_cast<float *>(&a.vis32[o]);
 #pragma unroll
-    for (int e = 0; e < 8; ++e) v32[e] += (float)acc[e];
+    for (int e = 0; e < 8; ++e) v32[e] += (float)two_level_sum(acc[0][e], tot[0][e]);
     if (a.vis64 != nullptr) {
         double *v64 = reinterpret_cast<double *>(&a.vis64[o]);
 #pragma unroll
-        for (int e = 0; e < 8; ++e) v64[e] += (double)acc[e];
+        for (int e = 0; e < 8; ++e) v64[e] += (double)two_level_sum(acc[0][e], tot[0][e]);
     }
 }
 

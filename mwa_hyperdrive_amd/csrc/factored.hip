@@ -109,15 +109,32 @@ __global__ __launch_bounds__(256) void k_factored_phasors(const FactoredArgs a) 
     const size_t fstride = (size_t)a.tpad * a.cpad;
     float2 *dst = a.kbuf + i;
     if (t >= a.ntiles || cl >= a.c_count || c >= a.ncomp) {
-        for (int f = 0; f < a.nchan; ++f) dst[f * fstride] = make_float2(0.0f, 0.0f);
+        for (int f = 0; f < a.nchan; ++f) {
+            dst[f * fstride] = make_float2(0.0f, 0.0f);
+            if (a.khi != nullptr) a.khi[i + f * fstride] = a.klo[i + f * fstride] = 0u;
+        }
         return;
     }
     const double4 l = a.lmn[c];
     const double th = a.tile_uvw[3 * t] * l.x + a.tile_uvw[3 * t + 1] * l.y + a.tile_uvw[3 * t + 2] * l.z;
+    // Stores K at channel f, and with planes its hi / lo split in the MFMA type.
+    auto put = [&](int f, double co, double s) {
+        const float re = (float)co, im = (float)s;
+        dst[f * fstride] = make_float2(re, im);
+        if (a.khi != nullptr) {
+            unsigned hi, lo;
+            if (a.planes_dtype == SM_MFMA_BF16)
+                split_words<__bf16>(re, im, hi, lo);
+            else
+                split_words<_Float16>(re, im, hi, lo);
+            a.khi[i + f * fstride] = hi;
+            a.klo[i + f * fstride] = lo;
+        }
+    };
     for (int f0 = 0; f0 < a.nchan; f0 += SM_NCH) {
         double s, co;
         fast_sincos(th * (a.freqs[f0] / SM_VEL_C), s, co);
-        dst[f0 * fstride] = make_float2((float)co, (float)s);
+        put(f0, co, s);
         if (a.even) {
             double ds, dc;
             fast_sincos(th * ((a.freqs[f0 + 1] - a.freqs[f0]) / SM_VEL_C), ds, dc);
@@ -125,12 +142,12 @@ __global__ __launch_bounds__(256) void k_factored_phasors(const FactoredArgs a) 
                 const double ns = s * dc + co * ds;
                 co = co * dc - s * ds;
                 s = ns;
-                dst[(f0 + k) * fstride] = make_float2((float)co, (float)s);
+                put(f0 + k, co, s);
             }
         } else {
             for (int k = 1; k < SM_NCH && f0 + k < a.nchan; ++k) {
                 fast_sincos(th * (a.freqs[f0 + k] / SM_VEL_C), s, co);
-                dst[(f0 + k) * fstride] = make_float2((float)co, (float)s);
+                put(f0 + k, co, s);
             }
         }
     }
@@ -608,6 +625,192 @@ __global__ __launch_bounds__(768) void k_factored_ws(const FactoredArgs a) {
         factored_ws_block<E, false, D>(a, buf, f, blk, half, pq);
 }
 
+// ---------------------------------------------------------------- K operand from planes
+// Kernel 2: the phasor kernel stores K already split into hi and lo halves of the MFMA type
+// (khi, klo), so the K operand is the same for all six work-groups of a channel and is not
+// regenerated by each. Four MFMA waves read their K fragments straight from those planes
+// (one 16-byte load per fragment, one step ahead), and four producer waves generate only the
+// A operand into LDS: half the LDS traffic of kernel 1 per step. Operands, products and their
+// order are those of kernels 0 and 1, so the results agree bit for bit.
+
+template <class E> struct StepLdsA {
+    E a[2][2][kFB * kFK];  // [hi / lo][entry]
+};
+
+// A producer thread's operands for one step: K_p of 4 components and their coherencies.
+struct OpsA {
+    float4 p[2], c[4];
+};
+
+template <class E>
+__device__ __forceinline__ void generate_a(const OpsA &g, StepLdsA<E> &L, int gr, int gq) {
+    const float kr[4] = {g.p[0].x, g.p[0].z, g.p[1].x, g.p[1].z};
+    const float ki[4] = {g.p[0].y, g.p[0].w, g.p[1].y, g.p[1].w};
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        uint4 h, l;
+        unsigned *hw = &h.x, *lw = &l.x;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float cr = e ? g.c[j].z : g.c[j].x;
+            const float ci = e ? g.c[j].w : g.c[j].y;
+            split_words<E>(cmul_re(kr[j], ki[j], cr, ci), cmul_im(kr[j], ki[j], cr, ci), hw[j], lw[j]);
+        }
+        *reinterpret_cast<uint4 *>(&L.a[0][e][lds_at(gr, 2 * gq)]) = h;
+        *reinterpret_cast<uint4 *>(&L.a[1][e][lds_at(gr, 2 * gq)]) = l;
+    }
+}
+
+// A step's K fragments of one MFMA wave: [K slice][column tile] hi and lo (8 values each).
+struct KFrags {
+    uint4 hi[2][2], lo[2][2];
+};
+
+template <class E, bool DIAG, int D>
+__device__ __forceinline__ void factored_gb_block(const FactoredArgs &a, StepLdsA<E> (&buf)[2], int f, int blk,
+                                                  int half, int2 pq) {
+    typedef typename Mfma<E>::vec V;
+    static_assert(D >= 2 && D % 2 == 0, "an even number of register sets");
+    static_assert(kFC == 16 && kFB == 64, "256 producer threads cover 64 rows x 16 components");
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6;
+    const int nsteps = a.cpad / kFC;
+    if (wave >= 4) {
+        // Producers: thread pt handles row pt / 4 and components 4 (pt % 4) .. + 3 of a step.
+        const int pt = tid - 256;
+        const int gr = pt >> 2;
+        const int gq = (pt & 3) * 4;
+        const float2 *kf = a.kbuf + (size_t)f * a.tpad * a.cpad;
+        const unsigned offp = (unsigned)((pq.x * kFB + gr) * a.cpad + gq);
+        const int rc = a.rclass != nullptr ? a.rclass[pq.x * kFB + gr] : 0;
+        const float4 *cf = a.cohf + ((size_t)f * a.nclass + rc) * a.cpad * 2 + gq * 2 + half;
+        auto load = [&](OpsA &g, int step) {
+            step = step < nsteps ? step : nsteps - 1;
+            const unsigned o = (unsigned)step * kFC;
+            g.p[0] = *reinterpret_cast<const float4 *>(kf + offp + o);
+            g.p[1] = *reinterpret_cast<const float4 *>(kf + offp + o + 2);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) g.c[j] = cf[2 * (o + j)];
+        };
+        OpsA g[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) load(g[j], j);
+        generate_a<E>(g[0], buf[0], gr, gq);
+        load(g[0], D);
+        for (int s = 0; s < nsteps; s += D) {
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                if (s + d >= nsteps) break;
+                __syncthreads();
+                const int j = (d + 1) % D;
+                generate_a<E>(g[j], buf[(d + 1) & 1], gr, gq);
+                load(g[j], s + d + 1 + D);
+            }
+        }
+        return;
+    }
+    // MFMA waves: wave w computes local Jones entry w % 2 for output columns (w / 2) * 64 ..
+    // + 63 (V_re columns carry K, V_im columns (-K_im, K_re)), 2 x 2 tiles of 32 x 32.
+    const int lane = tid & 63;
+    const int le = wave & 1;
+    const bool vim = (wave >> 1) != 0;
+    const int colbase = (wave >> 1) * kFB;
+    const int row = lane & 31;
+    const int kofs = (lane >> 5) * 8;
+    // K fragment of lane l: column tile t2 holds tile q = pq.y * 64 + t2 * 32 + l % 32; the
+    // 8 values are components 4 (l / 32) .. + 3 of the K slice, (re, im) each.
+    const size_t pf = (size_t)f * a.tpad * a.cpad;
+    const uint4 *khi = reinterpret_cast<const uint4 *>(a.khi + pf);
+    const uint4 *klo = reinterpret_cast<const uint4 *>(a.klo + pf);
+    unsigned qoff[2];
+#pragma unroll
+    for (int t2 = 0; t2 < 2; ++t2)
+        qoff[t2] = (unsigned)(((pq.y * kFB + t2 * 32 + row) * a.cpad + (lane >> 5) * 4) / 4);
+    auto load_k = [&](KFrags &K, int step) {
+        step = step < nsteps ? step : nsteps - 1;
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+            for (int t2 = 0; t2 < 2; ++t2) {
+                const unsigned o = qoff[t2] + (unsigned)(step * kFC + kb * 8) / 4;
+                K.hi[kb][t2] = khi[o];
+                K.lo[kb][t2] = klo[o];
+            }
+    };
+    auto as_v = [&](uint4 w) {
+        if (vim) {
+            w.x = swap_neg(w.x);
+            w.y = swap_neg(w.y);
+            w.z = swap_neg(w.z);
+            w.w = swap_neg(w.w);
+        }
+        return __builtin_bit_cast(V, w);
+    };
+    f16v acc[2][2], cor[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = cor[i][j][r] = 0.0f;
+    KFrags K[2];
+    load_k(K[0], 0);
+    for (int step = 0; step < nsteps; ++step) {
+        load_k(K[(step + 1) & 1], step + 1);
+        __syncthreads();
+        const KFrags &Kc = K[step & 1];
+#pragma unroll
+        for (int kb = 0; kb < kFK / 16; ++kb) {
+            Frags<E> F;
+#pragma unroll
+            for (int t2 = 0; t2 < 2; ++t2) {
+                const int ia = lds_at(t2 * 32 + row, kb * 16 + kofs);
+                F.ahi[t2] = *reinterpret_cast<const V *>(&buf[step & 1].a[0][le][ia]);
+                F.alo[t2] = *reinterpret_cast<const V *>(&buf[step & 1].a[1][le][ia]);
+                F.bhi[t2] = as_v(Kc.hi[kb][t2]);
+                F.blo[t2] = as_v(Kc.lo[kb][t2]);
+            }
+            mfma_rounds<E, DIAG>(F, acc, cor);
+        }
+        if ((step + 1) % a.seg_steps == 0 || step + 1 == nsteps) {
+            double *out = a.out + ((size_t)(f * a.nblocks + blk) * 4 + 2 * half) * (kFB * 2 * kFB);
+            unsigned lo = (unsigned)(le * (kFB * 2 * kFB) + (4 * (lane >> 5)) * (2 * kFB) + colbase + (lane & 31));
+            asm volatile("" : "+v"(lo));
+#pragma unroll
+            for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+                for (int ct = 0; ct < 2; ++ct) {
+                    if (DIAG && rt == 1 && ct == 0) continue;
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        const int o = (rt * 32 + 8 * (r >> 2) + (r & 3)) * (2 * kFB) + ct * 32;
+                        unsafeAtomicAdd(&out[lo + o], (double)acc[rt][ct][r] + (double)cor[rt][ct][r]);
+                        acc[rt][ct][r] = 0.0f;
+                        cor[rt][ct][r] = 0.0f;
+                        if ((r & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+                    }
+                }
+        }
+    }
+}
+
+template <class E, int D>
+__global__ __launch_bounds__(512) void k_factored_gb(const FactoredArgs a) {
+    __shared__ __attribute__((aligned(16))) StepLdsA<E> buf[2];
+    const int total = a.nchan * a.nblocks * 2;
+    int b = blockIdx.x;
+    if ((total & 7) == 0) b = (b & 7) * (total >> 3) + (b >> 3);
+    const int f = b / (2 * a.nblocks);
+    const int rem = b - f * 2 * a.nblocks;
+    const int blk = rem >> 1;
+    const int half = rem & 1;
+    const int2 pq = a.blocks[blk];
+    if (pq.x == pq.y)
+        factored_gb_block<E, true, D>(a, buf, f, blk, half, pq);
+    else
+        factored_gb_block<E, false, D>(a, buf, f, blk, half, pq);
+}
+
 // FP64 block results -> visibilities [freq][baseline] (+=), undoing the scale. map[i] =
 // (baseline, p, q): the output baseline and its two tiles in the factored tile numbering.
 __global__ void k_factored_scatter(const FactoredArgs a, const int4 *__restrict__ map, int nmap, int nbl,
@@ -748,7 +951,13 @@ hipError_t launch_factored(int dtype, FactoredArgs a, int chunk, const double *c
         } else {
             hipLaunchKernelGGL(k_factored_coh32, dim3((unsigned)((ncf + 255) / 256)), dim3(256), 0, stream, a);
         }
-        if (g_factored_kernel == 1) {
+        if (g_factored_kernel == 2) {
+            if (a.khi == nullptr || a.klo == nullptr || a.planes_dtype != dtype) return hipErrorInvalidValue;
+            if (dtype == SM_MFMA_BF16)
+                hipLaunchKernelGGL((k_factored_gb<__bf16, 4>), dim3(total), dim3(512), 0, stream, a);
+            else
+                hipLaunchKernelGGL((k_factored_gb<_Float16, 4>), dim3(total), dim3(512), 0, stream, a);
+        } else if (g_factored_kernel == 1) {
             if (dtype == SM_MFMA_BF16)
                 hipLaunchKernelGGL((k_factored_ws<__bf16, 4>), dim3(total), dim3(768), 0, stream, a);
             else

@@ -216,6 +216,7 @@ struct SmModeller {
     DevBuf d_uvw, d_vis32, d_vis64, d_accum, d_tile_uvw, d_pairs;
     // split-precision point path
     DevBuf d_fblocks, d_fblock_of, d_fout, d_fscratch, d_fkbuf, d_fcohf, d_fmap;
+    DevBuf d_fkhi, d_fklo;  // pre-split phasor planes (factored kernel 2)
     int nfblocks = 0;
     // Per-tile beams. Class 0 is the most common beam row; every other beam row is a class
     // of its own. Factored tiles: class-0 tiles first, padded to a 64-tile block boundary,
@@ -732,7 +733,7 @@ extern "C" const char *sm_set_factored_segment(int32_t steps) {
 }
 
 extern "C" const char *sm_set_factored_kernel(int32_t kernel) {
-    if (kernel != 0 && kernel != 1) return "factored kernel must be 0 or 1";
+    if (kernel < 0 || kernel > 2) return "factored kernel must be 0, 1 or 2";
     g_factored_kernel = kernel;
     return nullptr;
 }
@@ -847,12 +848,41 @@ static const char *model_timestep_device(SmModeller *m, double lst, double latit
                                     d.coh.as<double>(), 0));
             coh = d.coh.as<double>();
         } else {
-            // Baselines outside group 0: FP64 direct with per-tile beams.
+            // Baselines outside group 0 (pairs of two tiles outside beam class 0): FP64 with
+            // per-tile beams, on the own-beam tile-pair kernel when they are all the pairs of the
+            // listed tiles, else on the baseline kernel's subset form.
             ModelCtx rest = ctx;
             rest.d_bl_subset = m->d_g0_rest.as<int>();
             rest.nsub = m->g0_nrest;
             rest.nclass = 0;  // no baseline of the subset has a class-0 tile
-            if (m->g0_nrest > 0) {
+            const bool own_pairs = g_direct_variant == 0 && m->n_other >= 2 && ctx.d_other_tiles != nullptr &&
+                                   (long)m->g0_nrest == (long)m->n_other * (m->n_other - 1) / 2;
+            if (m->g0_nrest > 0 && own_pairs) {
+                if (const char *e = timed(ty, e0, e1)) return e;
+                SM_TRY(d.coh.reserve(sizeof(double) * 8 * (size_t)n * m->nchan_pad));
+                SM_TRY(launch_coherency(n, m->nchan, m->nchan_pad, ctx.d_freqs_pad, in.fb, ctx.beam, false,
+                                        d.coh.as<double>(), 0));
+                PairJonesArgs pj;
+                pj.ntiles_list = m->n_other;
+                pj.ntiles_bl = m->nt;
+                pj.nbl = m->nbl;
+                pj.nchan = m->nchan;
+                pj.nchan_pad = m->nchan_pad;
+                pj.ncomp = n;
+                pj.tiles = ctx.d_other_tiles;
+                pj.tile_uvw = tile_uvw;
+                pj.freqs = ctx.d_freqs_pad;
+                pj.lmn = in.lmn;
+                pj.shape = in.shape;
+                pj.coh = d.coh.as<double>();
+                pj.beam = ctx.beam;
+                pj.vis32 = d_vis_fb;
+                pj.vis64 = d_vis_fb_f64;
+                pj.chunks = ctx.chunks;
+                if (e0 != nullptr) (void)hipEventRecord(e0, 0);
+                SM_TRY(launch_pair_jones(kinds[ty], pj, 0));
+                if (e1 != nullptr) (void)hipEventRecord(e1, 0);
+            } else if (m->g0_nrest > 0) {
                 if (const char *e = timed(ty, e0, e1)) return e;
                 SM_TRY(run_type(kinds[ty], in, rest, d.coh, m->scratch, d_vis_fb, d_vis_fb_f64, 0, e0, e1));
             }
@@ -902,6 +932,14 @@ static const char *model_timestep_device(SmModeller *m, double lst, double latit
         const int chunk = factored_chunk(m->nchan, fnt, n, fa.seg_steps);
         SM_TRY(m->d_fkbuf.reserve(factored_kbuf_bytes(m->nchan, fnt, chunk)));
         fa.kbuf = m->d_fkbuf.as<float2>();
+        if (g_factored_kernel == 2) {
+            // The hi and lo planes take 4 bytes each per phasor, half the FP32 buffer each.
+            SM_TRY(m->d_fkhi.reserve(factored_kbuf_bytes(m->nchan, fnt, chunk) / 2));
+            SM_TRY(m->d_fklo.reserve(factored_kbuf_bytes(m->nchan, fnt, chunk) / 2));
+            fa.khi = m->d_fkhi.as<unsigned>();
+            fa.klo = m->d_fklo.as<unsigned>();
+            fa.planes_dtype = factored_dtype;
+        }
         SM_TRY(m->d_fcohf.reserve(factored_cohf_bytes(m->nchan, chunk, nclass)));
         fa.cohf = m->d_fcohf.as<float4>();
         SM_TRY(launch_factored(factored_dtype, fa, chunk, coh, (size_t)n * m->nchan_pad * 8, bits, scale, fmap, nmap,

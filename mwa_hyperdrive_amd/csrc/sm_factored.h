@@ -20,6 +20,10 @@ struct FactoredArgs {
     const double *coh;        // [ncomp][nchan_pad][8], J S J^H
     const double *scale;      // device scalar, set by launch_factored
     float2 *kbuf;             // [nchan][tpad][cpad] phasors of the chunk
+    // Kernel 2 only: the same phasors split into hi and lo halves of the MFMA input type,
+    // (re, im) packed in one 32-bit word, [nchan][tpad][cpad] each. Null otherwise.
+    unsigned *khi = nullptr, *klo = nullptr;
+    int planes_dtype = -1;    // SM_MFMA_FP16 / SM_MFMA_BF16 for the planes, -1 without them
     float4 *cohf;             // [nchan][nclass][cpad][2] scaled FP32 coherencies of the chunk
     double *out;              // [nchan][nblocks][4][64][128] FP64 block results
     // Per-tile beams: the A-side coherency of row tile p is J_r S J_0^H with r the beam row of
@@ -31,7 +35,8 @@ struct FactoredArgs {
     BeamDesc beam{};
 };
 
-// 1 (default): wave-specialised MFMA kernel; 0: every wave generates operands and runs MFMAs.
+// 1 (default): wave-specialised MFMA kernel; 0: every wave generates operands and runs MFMAs;
+// 2: wave-specialised with the K operand read by the MFMA waves from pre-split planes.
 extern int g_factored_kernel;
 
 std::vector<int2> factored_blocks(int ntiles, std::vector<int> &block_of);

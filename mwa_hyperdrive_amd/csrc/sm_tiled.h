@@ -68,7 +68,28 @@ struct RectArgs {
     ChunkTabs chunks;  // channel chunks; the fixed chunks must then be evenly spaced
 };
 
+// Pairs of two tiles with their own beam rows (tiles outside beam class 0): a block owns a
+// 16 x 16 block of tile pairs of the listed tiles and a chunk of channels, and per component
+// and channel stages J_p S in LDS for its 16 row tiles and J_q for its 16 column tiles, so
+// that each lane forms (J_p S) J_q^H from LDS instead of reading both tiles' Jones from
+// memory per visibility.
+struct PairJonesArgs {
+    int ntiles_list, ntiles_bl, nbl, nchan, nchan_pad, ncomp;
+    const int *tiles;         // [ntiles_list] array tiles, ascending
+    const double *tile_uvw;   // [ntiles_bl][3]
+    const double *freqs;      // [nchan_pad]
+    const double4 *lmn;
+    const double4 *shape;
+    const double *coh;        // [ncomp][nchan_pad][8] instrumental S
+    BeamDesc beam;            // per-tile rows: tile_rows, freq_map, jones
+    bool fp32 = false;        // FP32 arithmetic with FP64 phases
+    JonesF32 *vis32;
+    JonesF64 *vis64;
+    ChunkTabs chunks;
+};
+
 std::vector<int4> tile_pairs(int ntiles);
+hipError_t launch_pair_jones(Kind kind, const PairJonesArgs &a, hipStream_t stream);
 hipError_t launch_direct_rect(Kind kind, const RectArgs &a, hipStream_t stream);
 hipError_t launch_direct_tiled(Kind kind, const TiledArgs &a, hipStream_t stream);
 hipError_t launch_shapelet_chan(bool per_tile, const ShapeletChanArgs &a, hipStream_t stream);

@@ -120,9 +120,12 @@ def test_per_tile_beam_classes_fp64_and_split_vs_oracle():
     7 with a dead dipole, two pairs of them sharing the same dead dipole (one beam row each),
     all three component types. The FP64 path splits the baselines by beam-row class: pairs of
     two common-row tiles on the tile-pair kernel, pairs of a class-r tile and a common-row
-    tile on the shared-coherency baseline kernel (both orientations), the remaining pairs with
-    per-visibility J_p S J_q^H, and the shapelets through the per-class coherency table. Every
-    baseline must meet the FP64 bar; the split-precision path the fp16 bar."""
+    tile on the row-by-column kernel with J_r S J_0^H per class (both orientations), the
+    remaining pairs with per-visibility J_p S J_q^H, and the shapelets through the per-class
+    coherency table. With the class tables disabled (sm_set_class_table_limit(0)) the class-r
+    pairs take the shared-coherency baseline kernel per segment and the shapelets form
+    J_p S J_q^H per visibility; that path runs here too. Every baseline must meet the FP64 bar;
+    the split-precision path the fp16 bar."""
     base = synthetic.shrink(synthetic.config(2, scale=120 / 8000.0), tiles_n=40, chans=11, timesteps=1)
     n = 40
     delays = np.zeros((n, 16), np.uint32)
@@ -140,6 +143,14 @@ def test_per_tile_beam_classes_fp64_and_split_vs_oracle():
     _, _, v64 = m.model_timestep(float(base.lsts[0]), base.latitude, want_f64=True)
     err = rel_err(v64, o64)
     print(f"per-tile classes, FP64 direct: rel err {err:.3e}")
+    assert err <= 1e-12
+    try:
+        _lib.call("sm_set_class_table_limit", 0)
+        _, _, s64 = m.model_timestep(float(base.lsts[0]), base.latitude, want_f64=True)
+    finally:
+        _lib.call("sm_set_class_table_limit", -1)
+    err = rel_err(s64, o64)
+    print(f"per-tile classes, FP64 per-segment path: rel err {err:.3e}")
     assert err <= 1e-12
     shape = (len(base.freqs), n * (n - 1) // 2, 8)
     d32 = torch.zeros(shape, dtype=torch.float32, device="cuda")

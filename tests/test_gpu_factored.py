@@ -1,9 +1,11 @@
 """Split-precision MFMA point path (sm_model_timestep_factored).
 
 Bar: relative visibility error <= 1e-6 of max|V| against the FP64 oracle for fp16 hi+lo
-operands. bf16 hi+lo keeps 16 mantissa bits; its error is measured and reported, with a
-looser bound written here (5e-6). With per-tile beams, baselines outside the most common
-beam row take the FP64 direct kernel and must then meet the FP64 bar."""
+operands. bf16 hi+lo keeps 16 significand bits and does not meet that bar (measured 3.4e-6 on
+a C3 timestep): test_bf16_meets_the_1e6_bar states the bar for bf16 as an expected failure,
+and the other tests hold bf16 to its measured bound (BF16_MEASURED, 5e-6) so that a
+regression still shows. With per-tile beams, baselines outside the most common beam row take
+the FP64 direct kernel and must then meet the FP64 bar."""
 
 import ctypes as C
 
@@ -19,7 +21,7 @@ from tests.helpers import gpu_model, oracle_model, rel_err
 pytestmark = pytest.mark.gpu
 
 FP16_TOL = 1e-6
-BF16_TOL = 5e-6
+BF16_MEASURED = 5e-6  # bf16's measured error bound, not the bar
 
 
 def test_mfma_layout_selftest():
@@ -56,12 +58,22 @@ def test_factored_vs_oracle(beam, n_tiles):
         obs.delays = np.zeros((n_tiles, 16), np.uint32)
         obs.gains = np.ones((n_tiles, 32))
     o32, o64 = oracle_model(obs, 1)
-    for dtype, tol, name in [(0, FP16_TOL, "fp16"), (1, BF16_TOL, "bf16")]:
+    for dtype, tol, name in [(0, FP16_TOL, "fp16"), (1, BF16_MEASURED, "bf16")]:
         v32, v64, m = factored(obs, 1, dtype)
         err = rel_err(v64, o64)
         print(f"{name} beam={beam} tiles={n_tiles}: rel err {err:.3e}")
         assert err <= tol
         assert m.info()["num_points"] == obs.num_components
+
+
+@pytest.mark.xfail(reason="bf16 hi + lo carries 16 significand bits: about 3e-6, above the 1e-6 bar", strict=False)
+def test_bf16_meets_the_1e6_bar():
+    obs = synthetic.shrink(synthetic.config(3, scale=2000 / 20000), tiles_n=128, chans=16, timesteps=1)
+    o32, o64 = oracle_model(obs, 0)
+    _, v64, _ = factored(obs, 0, 1)
+    err = rel_err(v64, o64)
+    print(f"bf16 hi+lo, 2000 points, 128 tiles: rel err {err:.3e} against the 1e-6 bar")
+    assert err <= FP16_TOL
 
 
 @pytest.mark.parametrize("n_tiles,dead", [(20, [4]), (40, [3, 11, 25, 38]), (70, [0, 69]), (90, [1, 40, 64, 70, 89])])
@@ -114,7 +126,7 @@ def test_factored_c3_full_timestep_vs_fp64_direct():
     ref = torch.zeros(shape, dtype=torch.float64, device="cuda")
     d32 = torch.zeros(shape, dtype=torch.float32, device="cuda")
     m.model_timestep_with_device(float(obs.lsts[5]), obs.latitude, d32, ref)
-    for dtype, tol, name in [(0, FP16_TOL, "fp16"), (1, BF16_TOL, "bf16")]:
+    for dtype, tol, name in [(0, FP16_TOL, "fp16"), (1, BF16_MEASURED, "bf16")]:
         d64 = torch.zeros(shape, dtype=torch.float64, device="cuda")
         _lib.call("sm_model_timestep_factored", m._h, float(obs.lsts[5]), obs.latitude, dtype,
                   C.c_void_p(d32.data_ptr()), C.c_void_p(d64.data_ptr()))
@@ -128,9 +140,10 @@ def test_factored_c3_full_timestep_vs_fp64_direct():
 
 @pytest.mark.parametrize("n_comps", [10, 23, 40])
 def test_factored_kernel_organisations_agree(n_comps):
-    """Both work-group organisations of the split-precision kernel (every wave generates and
-    multiplies; 8 producer waves feed 4 MFMA waves) compute the same operands and products in
-    the same order: FP64 outputs equal bit for bit, for 1, 2 and 3 steps of 16 components,
+    """The three work-group organisations of the split-precision kernel (every wave generates
+    and multiplies; 8 producer waves feed 4 MFMA waves; 4 producer waves feed 4 MFMA waves that
+    read the K operand from pre-split planes) compute the same operands and products in the
+    same order: FP64 outputs equal bit for bit, for 1, 2 and 3 steps of 16 components,
     diagonal and off-diagonal tile blocks (70 tiles), and FP32 spans of 1, 3 and 64 steps."""
     obs = synthetic.shrink(synthetic.config(3, scale=n_comps / 20000), tiles_n=70, chans=5, timesteps=1)
     try:
@@ -138,10 +151,11 @@ def test_factored_kernel_organisations_agree(n_comps):
             for seg in (1, 3, 64):
                 _lib.call("sm_set_factored_segment", seg)
                 outs = []
-                for kernel in (0, 1):
+                for kernel in (0, 1, 2):
                     _lib.call("sm_set_factored_kernel", kernel)
                     outs.append(factored(obs, 0, dtype)[1])
                 assert np.array_equal(outs[0], outs[1]), (dtype, seg)
+                assert np.array_equal(outs[0], outs[2]), (dtype, seg, "planes")
                 assert np.isfinite(outs[0]).all()
     finally:
         _lib.call("sm_set_factored_segment", 64)

@@ -120,7 +120,10 @@ def test_training_reduces_error_and_fallback_is_exact():
 def test_modeller_with_surrogate_beam():
     """A C2-like mixed sky through the SkyModeller with the surrogate beam: threshold 0 gives
     the analytic-beam visibilities bit for bit (every tile falls back); an infinite threshold
-    uses the network, whose visibility error is reported against the FP64 oracle."""
+    uses the network on every tile (no fallback), and the visibility error against the FP64
+    oracle is bounded by the network's Jones error on the sky's own directions: each term
+    J_p S J_q^H moves by at most ||S|| (2 ||J|| d + d^2) (Frobenius norms, d = max ||J_sur -
+    J_exact||), phasors and Gaussian envelopes having modulus <= 1 (shapelets: sum |c_k|)."""
     from mwa_hyperdrive_amd import synthetic
     from mwa_hyperdrive_amd.modeller import AnalyticBeam, SkyModeller
     from tests.helpers import oracle_model, rel_err
@@ -137,8 +140,34 @@ def test_modeller_with_surrogate_beam():
     assert nfb >= 1
     np.testing.assert_array_equal(v32, ref32)
     w32, _, w64 = m.model_timestep(float(obs.lsts[0]), obs.latitude, want_f64=True)
-    assert m.set_beam_surrogate(None, 0.0) == 0
+    assert m.set_beam_surrogate(None, 0.0) == 0  # the network served every tile
     o32, o64 = oracle_model(obs, 0)
-    print(f"surrogate-beam sky: rel vis err vs oracle {rel_err(w64, o64):.3e}; exact beam {rel_err(ref64, o64):.3e}")
     assert rel_err(ref64, o64) <= 1e-12
+    # The network's Jones error on the directions and channels this sky uses.
+    from oracle import oracle as orc
+    from mwa_hyperdrive_amd.srclist import flat_components
+    comps = flat_components(obs.srclist)
+    lst = float(obs.lsts[0])
+    azel = [orc.azel(lst - c.ra, c.dec, obs.latitude) for c in comps]
+    az = torch.tensor([a for a, _ in azel], dtype=torch.float64, device="cuda")
+    za = torch.tensor([np.pi / 2 - e for _, e in azel], dtype=torch.float64, device="cuda")
+    d0, g0 = obs.delays[:1], obs.gains[:1]
+    exact = analytic_beam_device(d0, g0, obs.freqs, az, za)
+    sur, nfb = s.beam(d0, g0, obs.freqs, az, za, threshold=1e30)
+    assert nfb == 0
+    d = (sur - exact).pow(2).sum(-1).sqrt().max().item()
+    jmax = exact.pow(2).sum(-1).sqrt().max().item()
+    # ||S||_F = sqrt(2 (I^2 + Q^2 + U^2 + V^2)); a shapelet envelope is at most sum |c_k| (the
+    # basis functions e^{-x^2/2} H_n(x) / sqrt(2^n n!) are bounded by 1, Cramer's inequality).
+    from mwa_hyperdrive_amd.srclist import Shapelet
+    ssum = 0.0
+    for c in comps:
+        w = sum(abs(v) for _, _, v in c.comp_type.coeffs) if isinstance(c.comp_type, Shapelet) else 1.0
+        ssum += w * max(np.sqrt(2 * sum(x * x for x in orc.estimate_at_freq(c, f)[1:])) for f in obs.freqs)
+    scale = np.abs(o64[..., 0::2] + 1j * o64[..., 1::2]).max()
+    bound = ssum * (2 * jmax * d + d * d) / scale
+    err = rel_err(w64, o64)
+    print(f"surrogate-beam sky: rel vis err vs oracle {err:.3e} with the network on every tile (bound "
+          f"{bound:.3e} from max |dJ| {d:.3e}); exact beam {rel_err(ref64, o64):.3e}")
     assert np.isfinite(w64).all()
+    assert 0.0 < err <= bound

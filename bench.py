@@ -2,6 +2,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
                     [--no-split-precision] [--no-surrogate]
+    python bench.py --surrogate-sweep   # held-out accuracy against network size and epochs
 
 Headline workload (configuration C2): 128 tiles (8128 cross baselines), 768 channels x
 40 kHz from 167 MHz, 14 timesteps of 8 s, 8000 components (5600 points, 2000 Gaussians,
@@ -248,7 +249,7 @@ def split_precision() -> dict:
     return res
 
 
-def combined_c5() -> dict:
+def combined_c5(with_surrogate: bool = True) -> dict:
     """C5 (256 tiles, per-tile beams with dead dipoles, 50 000 points + 10 000 Gaussians + 500
     shapelets, 384 channels, 28 timesteps): every timestep through the FP64 direct path and
     through the precision path (fp16 points on MFMA by beam-row classes, Gaussians and shapelets
@@ -301,6 +302,10 @@ def combined_c5() -> dict:
     scale = ref.reshape(-1, 2).norm(dim=1).max().item()
     res["fp16_points_mfma"]["rel_err_vs_fp64"] = (out - ref).reshape(-1, 2).norm(dim=1).max().item() / scale
     res["speedup_end_to_end"] = res["fp64_direct"]["wall_ms"] / res["fp16_points_mfma"]["wall_ms"]
+    if not with_surrogate:
+        del m, v32, ref, out
+        torch.cuda.empty_cache()
+        return res
     # The same precision path with the beam from the MLP surrogate and the exact beam as per-tile
     # fallback above a 1e-3 Jones-error threshold (a short training run first).
     from mwa_hyperdrive_amd import surrogate as sg
@@ -409,6 +414,9 @@ def main() -> int:
     ap.add_argument("--probe", action="store_true",
                     help="counter runs only: one plain 1 GiB device copy (the calibration kernel), one C2 "
                          "timestep (FP64) and one C3 timestep (fp16 split precision), no JSON")
+    ap.add_argument("--surrogate-sweep", action="store_true",
+                    help="held-out Jones error of the beam surrogate against width, depth and epochs; "
+                         "one JSON line, no benchmark")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -423,6 +431,15 @@ def main() -> int:
     from mwa_hyperdrive_amd.modeller import AnalyticBeam, SkyModeller
 
     torch.cuda.set_device(0)
+    if args.surrogate_sweep:
+        from mwa_hyperdrive_amd import surrogate as sg
+
+        runs = ([(256, 4, e) for e in (4, 16, 64)] + [(w, 4, 16) for w in (64, 128, 512)]
+                + [(256, d, 16) for d in (2, 6)] + [(1024, 4, 8)])
+        rows = sg.accuracy_sweep(runs, log=lambda t: print(f"[sweep] {t}", file=sys.stderr, flush=True))
+        print(json.dumps({"surrogate_sweep": rows, "samples": 1 << 21, "batch": 4096,
+                          "delays": "pointing (mode 1)", "heldout": 100000}), flush=True)
+        return 0
     if args.probe:
         src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
         dst = torch.empty_like(src)
@@ -543,7 +560,7 @@ def main() -> int:
           file=sys.stderr, flush=True)
     if not args.no_split_precision:
         out["split_precision"] = split_precision()
-        out["combined_c5"] = combined_c5()
+        out["combined_c5"] = combined_c5(with_surrogate=not args.no_surrogate)
     if not args.no_surrogate:
         out["surrogate"] = surrogate_bench(obs)
     if not args.no_cpu_baseline:

@@ -10,9 +10,10 @@
  *  2. f64 variants of the same: identical inputs plus a nullable JonesF64 output
  *     that receives the f64 accumulators before demotion (the parity path).
  *  3. A handle-level SkyModeller (src/model/mod.rs:123-204, src/model/gpu.rs:150-1527):
- *     construction from a source list, per-timestep modelling from (LST, latitude),
- *     update_with_a_source, autos. Precession and Epoch->LST conversion stay with the
- *     caller (the trait's Epoch methods call marlu for that, src/model/cpu.rs:731-769).
+ *     construction from a source list, per-timestep modelling from a timestamp (GPS
+ *     seconds, with the DUT1 and precession setting of the handle, as the trait's Epoch
+ *     methods do through marlu, src/model/cpu.rs:731-769) or from (LST, latitude),
+ *     update_with_a_source, autos.
  *
  * All device work is enqueued on the HIP null stream of the calling thread's current
  * device; entry points that take host buffers synchronise before returning.
@@ -65,7 +66,7 @@ enum { SM_BEAM_NONE = 0, SM_BEAM_ANALYTIC = 1 };
 /* Polarisations (src/context/mod.rs:26-33), used by mask_pols (src/model/mod.rs:264-285). */
 enum { SM_POLS_XX_XY_YX_YY = 0, SM_POLS_XX = 1, SM_POLS_YY = 2, SM_POLS_XX_YY = 3, SM_POLS_XX_YY_XY = 4 };
 
-/* Arguments of new_sky_modeller (src/model/mod.rs:215-227) minus the Epoch-only ones. */
+/* Arguments of new_sky_modeller (src/model/mod.rs:215-227). */
 typedef struct SmObsParams {
     int32_t num_unflagged_tiles;
     const double *unflagged_tile_xyzs; /* host [num_unflagged_tiles][3], geodetic metres */
@@ -79,6 +80,8 @@ typedef struct SmObsParams {
     const uint32_t *dipole_delays;     /* host [total tiles][16] or NULL (all 0) */
     const double *dipole_gains;        /* host [total tiles][32] or NULL (all 1) */
     int32_t pols;                      /* SM_POLS_* */
+    double dut1;                       /* UT1 - UTC [s] for the timestamp entry points */
+    int32_t apply_precession;          /* nonzero: timestamps are precessed to J2000 */
 } SmObsParams;
 
 typedef struct SmModeller SmModeller;
@@ -185,6 +188,49 @@ const char *sm_set_class_table_limit(int64_t bytes);
 const char *sm_get_flux_params(SmModeller *m, int32_t comp_type, int32_t flux_type, int32_t index, int32_t chan,
                                JonesF64 *fd, double *si, double *q);
 
+/* ------------------------------------------------------------------ timestamps
+ * The SkyModeller methods take a hifitime::Epoch (src/model/mod.rs:141-203); these take its
+ * GPS seconds. The modeller turns a timestamp into (LST, cross UVWs, latitude) as
+ * get_lst_uvws_latitude does (src/model/cpu.rs:731-769, src/model/gpu.rs:1357-1398), with
+ * the dut1 and apply_precession of SmObsParams (new_sky_modeller's, src/model/mod.rs:215-227).
+ * With precession the LST and latitude are the J2000 coordinates of the zenith of date, the
+ * cross UVWs come from the precessed tile XYZs at the J2000 phase centre, and the shapelet
+ * UVs from the unprecessed XYZs at that LST, as the reference computes them
+ * (src/model/cpu.rs:782-785). */
+
+/* marlu 0.17.1 precession::PrecessionInfo, the fields the reference reads. rotation_matrix
+ * takes vectors in the true equator and equinox of date to J2000; hadec_j2000 is the
+ * aberrated phase centre at lmst_j2000. */
+typedef struct SmPrecessionInfo {
+    double rotation_matrix[3][3];
+    HADec hadec_j2000;
+    double lmst, lmst_j2000, array_latitude_j2000;
+} SmPrecessionInfo;
+
+/* marlu precession::get_lmst (src/model/cpu.rs:752): local mean sidereal time [rad], IAU 2006
+ * GMST at UT1 = UTC + dut1 plus the longitude. */
+double sm_get_lmst(double array_longitude_rad, double gps_seconds, double dut1_seconds);
+/* marlu precession::precess_time (src/model/cpu.rs:733-739). */
+const char *sm_precess_time(double array_longitude_rad, double array_latitude_rad, double pc_ra, double pc_dec,
+                            double gps_seconds, double dut1_seconds, SmPrecessionInfo *out);
+/* PrecessionInfo::precess_xyz (src/model/cpu.rs:741): host XYZs rotated into the J2000 frame. */
+const char *sm_precess_xyz(const SmPrecessionInfo *info, const XYZ *xyzs, XYZ *out, int32_t n);
+
+/* get_lst_uvws_latitude (src/model/cpu.rs:731-769) for the modeller's settings: the LST and
+ * latitude the timestamp models at, and (nullable, host) the cross UVWs in metres. */
+const char *sm_lst_uvws_latitude(SmModeller *m, double gps_seconds, double *lst, double *latitude, UVW *uvws_out);
+/* SkyModeller::model_timestep_with(Epoch) (src/model/mod.rs:156-163, src/model/cpu.rs:796-823):
+ * like sm_model_timestep_with, at a timestamp. */
+const char *sm_model_timestep_epoch_with(SmModeller *m, double gps_seconds, UVW *uvws_out, JonesF32 *vis_fb,
+                                         JonesF64 *vis_fb_f64);
+/* The device-buffer form (src/model/gpu.rs:1476-1496 without the copies): d_vis_fb /
+ * d_vis_fb_f64 are device memory, accumulated into; returns once the work is enqueued. */
+const char *sm_model_timestep_epoch_with_device(SmModeller *m, double gps_seconds, JonesF32 *d_vis_fb,
+                                                JonesF64 *d_vis_fb_f64);
+/* SkyModeller::model_timestep_autos_with(Epoch) (src/model/mod.rs:195-203). */
+const char *sm_model_timestep_autos_epoch_with(SmModeller *m, double gps_seconds, JonesF32 *vis_fb,
+                                               JonesF64 *vis_fb_f64);
+
 /* ------------------------------------------------------------------ peel-loop neighbours
  * The device work around the modeller in the reference's peel loop
  * (src/params/peel/gpu.rs:441-470). Same signatures as src/gpu/peel.h:13-35 with
@@ -224,6 +270,13 @@ const char *sm_analytic_beam(int32_t n_tiles, const uint32_t *delays, const doub
 typedef struct SmSurrogate SmSurrogate;
 int32_t sm_surrogate_num_params(void);
 const char *sm_surrogate_new(uint64_t seed, SmSurrogate **out);
+/* A network of `depth` hidden layers (1..8) of `hidden` SiLU units (1..4096), for studies of
+ * what accuracy a size reaches. Shapes other than 4 x 256 train and evaluate in FP32: the
+ * FP16 forward (dtype 1) is the fused 4 x 256 kernel, and sm_surrogate_beam uses FP32 for
+ * them. Parameters are flat like the default network's, layer by layer. */
+const char *sm_surrogate_new_shaped(uint64_t seed, int32_t hidden, int32_t depth, SmSurrogate **out);
+/* The network's hidden width, depth and parameter count. */
+const char *sm_surrogate_shape(const SmSurrogate *s, int32_t *hidden, int32_t *depth, int64_t *num_params);
 void sm_surrogate_free(SmSurrogate *s);
 const char *sm_surrogate_get_params(SmSurrogate *s, float *host_params);
 const char *sm_surrogate_set_params(SmSurrogate *s, const float *host_params);

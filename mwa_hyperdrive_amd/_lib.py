@@ -107,6 +107,26 @@ class SmObsParams(C.Structure):
         ("dipole_delays", C.c_void_p),
         ("dipole_gains", C.c_void_p),
         ("pols", C.c_int32),
+        ("dut1", C.c_double),
+        ("apply_precession", C.c_int32),
+    ]
+
+
+class HADec(C.Structure):
+    _fields_ = [("ha", C.c_double), ("dec", C.c_double)]
+
+
+class XYZ(C.Structure):
+    _fields_ = [("x", C.c_double), ("y", C.c_double), ("z", C.c_double)]
+
+
+class SmPrecessionInfo(C.Structure):
+    _fields_ = [
+        ("rotation_matrix", (C.c_double * 3) * 3),
+        ("hadec_j2000", HADec),
+        ("lmst", C.c_double),
+        ("lmst_j2000", C.c_double),
+        ("array_latitude_j2000", C.c_double),
     ]
 
 
@@ -161,6 +181,8 @@ SIGNATURES = {
     "sm_set_factored_segment": [C.c_int32],
     "sm_set_factored_kernel": [C.c_int32],
     "sm_surrogate_new": [C.c_uint64, C.POINTER(_P)],
+    "sm_surrogate_new_shaped": [C.c_uint64, _I, _I, C.POINTER(_P)],
+    "sm_surrogate_shape": [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(C.c_int64)],
     "sm_surrogate_get_params": [_P, _P],
     "sm_surrogate_set_params": [_P, _P],
     "sm_surrogate_samples": [C.c_uint64, C.c_int64, _I, _I, _P, _P, _P],
@@ -169,7 +191,15 @@ SIGNATURES = {
     "sm_surrogate_forward": [_P, _I, _P, _I, _P],
     "sm_surrogate_beam": [_P, _I, _P, _P, _I, _P, _I, _P, _P, _D, _P, C.POINTER(_I)],
     "sm_set_beam_surrogate": [_P, _P, _D, C.POINTER(_I)],
+    "sm_precess_time": [_D, _D, _D, _D, _D, _D, C.POINTER(SmPrecessionInfo)],
+    "sm_precess_xyz": [C.POINTER(SmPrecessionInfo), _P, _P, _I],
+    "sm_lst_uvws_latitude": [_P, _D, C.POINTER(_D), C.POINTER(_D), _P],
+    "sm_model_timestep_epoch_with": [_P, _D, _P, _P, _P],
+    "sm_model_timestep_epoch_with_device": [_P, _D, _P, _P],
+    "sm_model_timestep_autos_epoch_with": [_P, _D, _P, _P],
 }
+# Entry points that return a value rather than an error string.
+VALUE_RETURN = {"sm_get_lmst": ([_D, _D, _D], C.c_double), "sm_surrogate_num_params": ([], C.c_int32)}
 VOID_RETURN = {"sm_free", "sm_surrogate_free"}
 
 
@@ -189,8 +219,9 @@ def lib():
         for name in VOID_RETURN:
             getattr(L, name).argtypes = [_P]
             getattr(L, name).restype = None
-        L.sm_surrogate_num_params.argtypes = []
-        L.sm_surrogate_num_params.restype = C.c_int32
+        for name, (args, res) in VALUE_RETURN.items():
+            getattr(L, name).argtypes = args
+            getattr(L, name).restype = res
         _lib = L
     return _lib
 

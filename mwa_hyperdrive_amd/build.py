@@ -19,7 +19,8 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libskymodel_hip.so"
-SOURCES = ["direct.hip", "direct_tiled.hip", "abi.hip", "modeller.hip", "factored.hip", "surrogate.hip", "peel.hip"]
+SOURCES = ["direct.hip", "direct_tiled.hip", "abi.hip", "modeller.hip", "factored.hip", "surrogate.hip", "peel.hip",
+           "precession.cpp"]
 ARCH = os.environ.get("SM_OFFLOAD_ARCH", "gfx950")
 
 

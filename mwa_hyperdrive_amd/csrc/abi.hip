@@ -304,7 +304,7 @@ hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf
             pj.vis32 = vis32;
             pj.vis64 = vis64;
             pj.chunks = ctx.chunks;
-            e = launch_pair_jones(kind, pj, stream);
+            e = launch_pair_jones(kind, pj, &scratch.accum, stream);
         } else if (e == hipSuccess && cnt > 0) {
             DirectArgs u = a;
             u.bl_subset = ctx.d_notg0_bl + off;

@@ -335,11 +335,18 @@ __global__ __launch_bounds__(256) void k_pair_jones(const PairJonesArgs a, int n
     __shared__ double s_uvw[32][3];
     __shared__ int s_row[32];
     const int ngpairs = ngroups * (ngroups + 1) / 2;
-    const int chunk = blockIdx.x / ngpairs;
+    // Block = (component slice, channel chunk, tile-group pair), the pair fastest.
+    const int per_slice = ngpairs * a.nchunks;
+    const int slice = blockIdx.x / per_slice;
+    const int in_slice = blockIdx.x - slice * per_slice;
+    const int chunk = in_slice / ngpairs;
+    const int slice_len = (a.ncomp + a.csplit - 1) / a.csplit;
+    const int c_begin = slice * slice_len;
+    const int c_end = a.ncomp < c_begin + slice_len ? a.ncomp : c_begin + slice_len;
     // Tile-group pair (g0 <= g1) of this block, in row-major order of the upper triangle.
     int2 gp = make_int2(0, 0);
     {
-        int rem = blockIdx.x - chunk * ngpairs, g0 = 0;
+        int rem = in_slice - chunk * ngpairs, g0 = 0;
         while (rem >= ngroups - g0) {
             rem -= ngroups - g0;
             ++g0;
@@ -373,8 +380,8 @@ __global__ __launch_bounds__(256) void k_pair_jones(const PairJonesArgs a, int n
 #pragma unroll
         for (int e = 0; e < 8; ++e) acc[k][e] = tot[k][e] = T(0);
 
-    for (int c0 = 0; c0 < a.ncomp; c0 += kPJC) {
-        const int nc = a.ncomp - c0 < kPJC ? a.ncomp - c0 : kPJC;
+    for (int c0 = c_begin; c0 < c_end; c0 += kPJC) {
+        const int nc = c_end - c0 < kPJC ? c_end - c0 : kPJC;
         {
             // One phasor pair per (component, tile slot): 8 x 32 = 256 threads.
             const int cc = tid >> 5, slot = tid & 31;
@@ -468,6 +475,20 @@ __global__ __launch_bounds__(256) void k_pair_jones(const PairJonesArgs a, int n
         __syncthreads();
     }
     if (!valid) return;
+    if (a.csplit > 1) {
+        // This slice's sums into accum[f][pair][8]; pairs in upper-triangle order of the list.
+        const int n = a.ntiles_list;
+        const size_t npairs = (size_t)n * (n - 1) / 2;
+        const size_t t = (size_t)ip * n - (size_t)ip * (ip + 1) / 2 + (size_t)(iq - ip - 1);
+#pragma unroll
+        for (int k = 0; k < kPJN; ++k) {
+            if (k >= cnt) break;
+            double *dst = a.accum + ((size_t)(f0 + k) * npairs + t) * 8;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) unsafeAtomicAdd(&dst[e], (double)two_level_sum(acc[k][e], tot[k][e]));
+        }
+        return;
+    }
     const int p = a.tiles[ip], q = a.tiles[iq];
     const size_t nt = (size_t)a.ntiles_bl;
     const size_t bl = (size_t)p * nt - (size_t)p * (p + 1) / 2 + (size_t)(q - p - 1);
@@ -486,11 +507,59 @@ __global__ __launch_bounds__(256) void k_pair_jones(const PairJonesArgs a, int n
     }
 }
 
-hipError_t launch_pair_jones(Kind kind, const PairJonesArgs &a, hipStream_t stream) {
+// vis[f][bl(pair)] += accum[f][pair] (FP64, and its FP32 rounding), pairs in upper-triangle
+// order of the tile list.
+__global__ void k_pair_finish(int nchan, int n, const int *__restrict__ tiles, int ntiles_bl, int nbl,
+                              const double *__restrict__ accum, JonesF32 *vis32, JonesF64 *vis64) {
+    const size_t npairs = (size_t)n * (n - 1) / 2;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)nchan * npairs) return;
+    const int f = (int)(i / npairs);
+    int ip, iq;
+    triangle_pair((int)(i - (size_t)f * npairs), n, ip, iq);
+    const size_t p = (size_t)tiles[ip], q = (size_t)tiles[iq];
+    const size_t o = (size_t)f * nbl + p * ntiles_bl - p * (p + 1) / 2 + (q - p - 1);
+    const double *src = accum + i * 8;
+    float *v32 = reinterpret_cast<float *>(&vis32[o]);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v32[e] += (float)src[e];
+    if (vis64 != nullptr) {
+        double *v64 = reinterpret_cast<double *>(&vis64[o]);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v64[e] += src[e];
+    }
+}
+
+__global__ void k_zero_accum(size_t n, double *x) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        x[i] = 0.0;
+}
+
+hipError_t launch_pair_jones(Kind kind, const PairJonesArgs &a_in, DevBuf *accum, hipStream_t stream) {
+    PairJonesArgs a = a_in;
     if (a.ncomp <= 0 || a.ntiles_list < 2 || a.nchan <= 0) return hipSuccess;
     if (kind == KIND_SHAPELET) return hipErrorInvalidValue;
     const int G = (a.ntiles_list + 15) / 16;
-    const long total = (long)(G * (G + 1) / 2) * chunk_count(a.chunks, kPJN, a.nchan);
+    a.nchunks = chunk_count(a.chunks, kPJN, a.nchan);
+    const long blocks = (long)(G * (G + 1) / 2) * a.nchunks;
+    // About 2048 blocks (8 per CU), each slice at least 64 components.
+    long split = (2048 + blocks - 1) / blocks;
+    split = std::min<long>(split, std::max(1, a.ncomp / (8 * kPJC)));
+    split = std::max(1L, std::min(split, 64L));
+    const size_t npairs = (size_t)a.ntiles_list * (a.ntiles_list - 1) / 2;
+    const size_t nacc = (size_t)a.nchan * npairs * 8;
+    a.csplit = 1;
+    a.accum = nullptr;
+    if (split > 1 && accum != nullptr) {
+        if (accum->reserve(sizeof(double) * nacc) == hipSuccess) {
+            a.csplit = (int)split;
+            a.accum = accum->as<double>();
+            hipLaunchKernelGGL(k_zero_accum, dim3(1024), dim3(256), 0, stream, nacc, a.accum);
+        } else {
+            (void)hipGetLastError();  // no room for the partial sums: unsplit launch
+        }
+    }
+    const long total = blocks * a.csplit;
     if (a.fp32) {
         if (kind == KIND_GAUSSIAN)
             hipLaunchKernelGGL((k_pair_jones<KIND_GAUSSIAN, float>), dim3((unsigned)total), dim3(256), 0, stream, a, G);
@@ -500,6 +569,11 @@ hipError_t launch_pair_jones(Kind kind, const PairJonesArgs &a, hipStream_t stre
         hipLaunchKernelGGL((k_pair_jones<KIND_GAUSSIAN, double>), dim3((unsigned)total), dim3(256), 0, stream, a, G);
     } else {
         hipLaunchKernelGGL((k_pair_jones<KIND_POINT, double>), dim3((unsigned)total), dim3(256), 0, stream, a, G);
+    }
+    if (a.csplit > 1) {
+        const size_t n = (size_t)a.nchan * npairs;
+        hipLaunchKernelGGL(k_pair_finish, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, a.nchan,
+                           a.ntiles_list, a.tiles, a.ntiles_bl, a.nbl, a.accum, a.vis32, a.vis64);
     }
     return hipGetLastError();
 }

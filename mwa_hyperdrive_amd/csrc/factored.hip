@@ -753,12 +753,9 @@ __device__ __forceinline__ void factored_gb_block(const FactoredArgs &a, StepLds
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = cor[i][j][r] = 0.0f;
-    KFrags K[2];
-    load_k(K[0], 0);
-    for (int step = 0; step < nsteps; ++step) {
-        load_k(K[(step + 1) & 1], step + 1);
-        __syncthreads();
-        const KFrags &Kc = K[step & 1];
+    // One step: the MFMA rounds on the A operand in LDS and the K fragments in Kc, then the
+    // FP64 flush at the end of a segment.
+    auto run_step = [&](const KFrags &Kc, int step) {
 #pragma unroll
         for (int kb = 0; kb < kFK / 16; ++kb) {
             Frags<E> F;
@@ -791,6 +788,20 @@ __device__ __forceinline__ void factored_gb_block(const FactoredArgs &a, StepLds
                     }
                 }
         }
+    };
+    // Two named register sets, the loop unrolled by two: a set indexed by the step's parity
+    // would be a dynamically indexed private array, which lives in scratch memory. One barrier
+    // per step, as the producer waves expect.
+    KFrags K0, K1;
+    load_k(K0, 0);
+    for (int step = 0; step < nsteps; step += 2) {
+        load_k(K1, step + 1);
+        __syncthreads();
+        run_step(K0, step);
+        if (step + 1 >= nsteps) break;
+        load_k(K0, step + 2);
+        __syncthreads();
+        run_step(K1, step + 1);
     }
 }
 

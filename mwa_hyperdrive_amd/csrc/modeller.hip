@@ -195,6 +195,9 @@ struct SmModeller {
     std::vector<double> xyz, freqs, freqs_pad;
     std::vector<int> flagged;
     double pc_ra = 0, pc_dec = 0, lon = 0, lat = 0;
+    double dut1 = 0.0;      // UT1 - UTC [s] for the timestamp entry points
+    bool precess = false;   // timestamps precessed to J2000
+    std::vector<double> xyz_j2000;  // precessed tile XYZs of the last timestamp call
     int beam_type = SM_BEAM_NONE, pols = SM_POLS_XX_XY_YX_YY;
     SmSurrogate *surrogate = nullptr;  // beam surrogate with exact fallback (not owned)
     double surrogate_threshold = 0.0;
@@ -508,6 +511,8 @@ extern "C" const char *sm_new(const SmObsParams *p, const SmComponent *comps, co
     m->pc_dec = p->phase_centre_dec;
     m->lon = p->array_longitude;
     m->lat = p->array_latitude;
+    m->dut1 = p->dut1;
+    m->precess = p->apply_precession != 0;
     m->beam_type = p->beam_type;
     m->pols = p->pols;
     m->delays.assign((size_t)m->n_total * 16, 0u);
@@ -740,14 +745,18 @@ extern "C" const char *sm_set_factored_kernel(int32_t kernel) {
 
 // One timestep on the device. factored_dtype < 0: FP64 direct kernels for every type;
 // otherwise points go through the split-precision MFMA path in that operand type.
+// uvw_xyz (nullable): the tile XYZs the cross UVWs come from, when not the modeller's own
+// (the precessed XYZs of a timestamp call); shapelet UVs always use the modeller's own.
 static const char *model_timestep_device(SmModeller *m, double lst, double latitude, const UVW *d_uvws,
-                                         JonesF32 *d_vis_fb, JonesF64 *d_vis_fb_f64, int factored_dtype) {
+                                         JonesF32 *d_vis_fb, JonesF64 *d_vis_fb_f64, int factored_dtype,
+                                         const std::vector<double> *uvw_xyz = nullptr) {
     if (m == nullptr || d_vis_fb == nullptr) return "null argument";
     if (m->nbl == 0) return nullptr;
     const UVW *uvw = d_uvws;
     const double *tile_uvw = nullptr;
     if (uvw == nullptr) {
-        cross_uvws(m->xyz, m->nt, lst - m->pc_ra, m->pc_dec, m->h_uvw, &m->h_tile_uvw);
+        cross_uvws(uvw_xyz != nullptr ? *uvw_xyz : m->xyz, m->nt, lst - m->pc_ra, m->pc_dec, m->h_uvw,
+                   &m->h_tile_uvw);
         SM_TRY(upload(m->d_uvw, m->h_uvw));
         SM_TRY(upload(m->d_tile_uvw, m->h_tile_uvw));
         uvw = m->d_uvw.as<UVW>();
@@ -880,7 +889,7 @@ static const char *model_timestep_device(SmModeller *m, double lst, double latit
                 pj.vis64 = d_vis_fb_f64;
                 pj.chunks = ctx.chunks;
                 if (e0 != nullptr) (void)hipEventRecord(e0, 0);
-                SM_TRY(launch_pair_jones(kinds[ty], pj, 0));
+                SM_TRY(launch_pair_jones(kinds[ty], pj, &m->scratch.accum, 0));
                 if (e1 != nullptr) (void)hipEventRecord(e1, 0);
             } else if (m->g0_nrest > 0) {
                 if (const char *e = timed(ty, e0, e1)) return e;
@@ -959,11 +968,12 @@ extern "C" const char *sm_model_timestep_factored(SmModeller *m, double lst, dou
     return model_timestep_device(m, lst, latitude, nullptr, d_vis_fb, d_vis_fb_f64, dtype);
 }
 
-extern "C" const char *sm_model_timestep_with(SmModeller *m, double lst, double latitude, UVW *uvws_out,
-                                              JonesF32 *vis_fb, JonesF64 *vis_fb_f64) {
+// The host-buffer form over model_timestep_device: copies in, models, copies out, masks.
+static const char *model_timestep_host(SmModeller *m, double lst, double latitude, const std::vector<double> *uvw_xyz,
+                                       UVW *uvws_out, JonesF32 *vis_fb, JonesF64 *vis_fb_f64) {
     if (m == nullptr || vis_fb == nullptr) return "null argument";
     const size_t nvis = (size_t)m->nchan * m->nbl;
-    cross_uvws(m->xyz, m->nt, lst - m->pc_ra, m->pc_dec, m->h_uvw);
+    cross_uvws(uvw_xyz != nullptr ? *uvw_xyz : m->xyz, m->nt, lst - m->pc_ra, m->pc_dec, m->h_uvw);
     if (uvws_out != nullptr) std::memcpy(uvws_out, m->h_uvw.data(), sizeof(UVW) * m->h_uvw.size());
     if (nvis == 0) return nullptr;
     SM_TRY(m->d_vis32.reserve(sizeof(JonesF32) * nvis));
@@ -974,13 +984,85 @@ extern "C" const char *sm_model_timestep_with(SmModeller *m, double lst, double 
         SM_TRY(hipMemcpy(m->d_vis64.ptr, vis_fb_f64, sizeof(JonesF64) * nvis, hipMemcpyHostToDevice));
         d64 = m->d_vis64.as<JonesF64>();
     }
-    if (const char *e = sm_model_timestep_with_device(m, lst, latitude, nullptr, m->d_vis32.as<JonesF32>(), d64))
+    if (const char *e = model_timestep_device(m, lst, latitude, nullptr, m->d_vis32.as<JonesF32>(), d64, -1, uvw_xyz))
         return e;
     SM_TRY(hipMemcpy(vis_fb, m->d_vis32.ptr, sizeof(JonesF32) * nvis, hipMemcpyDeviceToHost));
     if (vis_fb_f64 != nullptr)
         SM_TRY(hipMemcpy(vis_fb_f64, m->d_vis64.ptr, sizeof(JonesF64) * nvis, hipMemcpyDeviceToHost));
     mask_pols(m->pols, vis_fb, vis_fb_f64, nvis);
     return nullptr;
+}
+
+extern "C" const char *sm_model_timestep_with(SmModeller *m, double lst, double latitude, UVW *uvws_out,
+                                              JonesF32 *vis_fb, JonesF64 *vis_fb_f64) {
+    return model_timestep_host(m, lst, latitude, nullptr, uvws_out, vis_fb, vis_fb_f64);
+}
+
+// get_lst_uvws_latitude (src/model/cpu.rs:731-769): the LST and latitude a timestamp models
+// at, and the tile XYZs its cross UVWs come from: with precession, the J2000 coordinates of
+// the zenith of date and the precessed XYZs (kept in m->xyz_j2000).
+static const char *lst_latitude_xyz(SmModeller *m, double gps_seconds, double &lst, double &latitude,
+                                    const std::vector<double> *&xyz) {
+    if (!std::isfinite(gps_seconds)) return "non-finite timestamp";
+    if (!m->precess) {
+        lst = sm_get_lmst(m->lon, gps_seconds, m->dut1);
+        latitude = m->lat;
+        xyz = &m->xyz;
+        return nullptr;
+    }
+    SmPrecessionInfo info;
+    if (const char *e = sm_precess_time(m->lon, m->lat, m->pc_ra, m->pc_dec, gps_seconds, m->dut1, &info)) return e;
+    m->xyz_j2000.resize(m->xyz.size());
+    if (const char *e = sm_precess_xyz(&info, reinterpret_cast<const XYZ *>(m->xyz.data()),
+                                       reinterpret_cast<XYZ *>(m->xyz_j2000.data()), m->nt))
+        return e;
+    lst = info.lmst_j2000;
+    latitude = info.array_latitude_j2000;
+    xyz = &m->xyz_j2000;
+    return nullptr;
+}
+
+extern "C" const char *sm_lst_uvws_latitude(SmModeller *m, double gps_seconds, double *lst, double *latitude,
+                                            UVW *uvws_out) {
+    if (m == nullptr) return "null argument";
+    double l = 0.0, b = 0.0;
+    const std::vector<double> *xyz = nullptr;
+    if (const char *e = lst_latitude_xyz(m, gps_seconds, l, b, xyz)) return e;
+    if (lst != nullptr) *lst = l;
+    if (latitude != nullptr) *latitude = b;
+    if (uvws_out != nullptr && m->nbl > 0) {
+        std::vector<UVW> u;
+        cross_uvws(*xyz, m->nt, l - m->pc_ra, m->pc_dec, u);
+        std::memcpy(uvws_out, u.data(), sizeof(UVW) * u.size());
+    }
+    return nullptr;
+}
+
+extern "C" const char *sm_model_timestep_epoch_with(SmModeller *m, double gps_seconds, UVW *uvws_out,
+                                                    JonesF32 *vis_fb, JonesF64 *vis_fb_f64) {
+    if (m == nullptr || vis_fb == nullptr) return "null argument";
+    double lst = 0.0, lat = 0.0;
+    const std::vector<double> *xyz = nullptr;
+    if (const char *e = lst_latitude_xyz(m, gps_seconds, lst, lat, xyz)) return e;
+    return model_timestep_host(m, lst, lat, xyz, uvws_out, vis_fb, vis_fb_f64);
+}
+
+extern "C" const char *sm_model_timestep_epoch_with_device(SmModeller *m, double gps_seconds, JonesF32 *d_vis_fb,
+                                                           JonesF64 *d_vis_fb_f64) {
+    if (m == nullptr || d_vis_fb == nullptr) return "null argument";
+    double lst = 0.0, lat = 0.0;
+    const std::vector<double> *xyz = nullptr;
+    if (const char *e = lst_latitude_xyz(m, gps_seconds, lst, lat, xyz)) return e;
+    return model_timestep_device(m, lst, lat, nullptr, d_vis_fb, d_vis_fb_f64, -1, xyz);
+}
+
+extern "C" const char *sm_model_timestep_autos_epoch_with(SmModeller *m, double gps_seconds, JonesF32 *vis_fb,
+                                                          JonesF64 *vis_fb_f64) {
+    if (m == nullptr || vis_fb == nullptr) return "null argument";
+    double lst = 0.0, lat = 0.0;
+    const std::vector<double> *xyz = nullptr;
+    if (const char *e = lst_latitude_xyz(m, gps_seconds, lst, lat, xyz)) return e;
+    return sm_model_timestep_autos_with(m, lst, lat, vis_fb, vis_fb_f64);
 }
 
 extern "C" const char *sm_model_timestep_autos_with(SmModeller *m, double lst, double latitude, JonesF32 *vis_fb,

@@ -16,21 +16,6 @@ namespace {
 
 constexpr double kTauPeel = 6.283185307179586;
 
-// Tiles (i, j), i < j, of upper-triangle baseline b over n tiles: row i holds n - 1 - i
-// baselines starting at i n - i (i + 1) / 2. The float estimate is corrected by integer tests,
-// so the result is exact for every n.
-__device__ __forceinline__ void baseline_tiles(int b, int n, int &i, int &j) {
-    const double nn = (double)n - 0.5;
-    int r = (int)floor(nn - sqrt(nn * nn - 2.0 * (double)b));
-    if (r < 0) r = 0;
-    if (r > n - 2) r = n - 2;
-    auto start = [n](int row) { return row * n - row * (row + 1) / 2; };
-    while (r > 0 && start(r) > b) --r;
-    while (r < n - 2 && start(r + 1) <= b) ++r;
-    i = r;
-    j = b - start(r) + r + 1;
-}
-
 // xyzs_to_uvws_kernel (src/gpu/peel.cu:17-49): per baseline, the tiles' XYZ difference
 // rotated to the pointing centre's (HA, Dec) at each timestep's LMST.
 __global__ void k_xyzs_to_uvws(const XYZ *__restrict__ xyzs, const double *__restrict__ lmsts, UVW *__restrict__ uvws,
@@ -38,7 +23,7 @@ __global__ void k_xyzs_to_uvws(const XYZ *__restrict__ xyzs, const double *__res
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nbl) return;
     int t1, t2;
-    baseline_tiles(b, ntiles, t1, t2);
+    triangle_pair(b, ntiles, t1, t2);
     double s_dec, c_dec;
     sincos(pc.dec, &s_dec, &c_dec);
     for (int t = 0; t < ntimes; ++t) {

@@ -95,6 +95,22 @@ __device__ __forceinline__ void chunk_range(const ChunkTabs &t, int chunk, int n
     }
 }
 
+// Index (i, j), i < j, of entry b of the upper triangle of an n x n grid in row-major order
+// (the baseline order of src/math/mod.rs:96-109): row i holds n - 1 - i entries starting at
+// i n - i (i + 1) / 2. The float estimate is corrected by integer tests, so the result is
+// exact for every n.
+__device__ __forceinline__ void triangle_pair(int b, int n, int &i, int &j) {
+    const double nn = (double)n - 0.5;
+    int r = (int)floor(nn - sqrt(nn * nn - 2.0 * (double)b));
+    if (r < 0) r = 0;
+    if (r > n - 2) r = n - 2;
+    auto start = [n](int row) { return row * n - row * (row + 1) / 2; };
+    while (r > 0 && start(r) > b) --r;
+    while (r < n - 2 && start(r + 1) <= b) ++r;
+    i = r;
+    j = b - start(r) + r + 1;
+}
+
 // A shapelet coefficient as the kernels read it: the value with the sign of
 // i^((n1 + n2) mod 4) folded in, and the basis-table rows n1, n2. Per component the terms
 // that land in the real part of the envelope come first, then those that land in the

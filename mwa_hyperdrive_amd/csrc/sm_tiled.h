@@ -86,10 +86,17 @@ struct PairJonesArgs {
     JonesF32 *vis32;
     JonesF64 *vis64;
     ChunkTabs chunks;
+    // Set by the launcher: channel chunks, and the component split. With csplit > 1 each
+    // block sums one slice of the components into accum [nchan][pairs][8] (FP64), and a
+    // finishing kernel adds those sums to the visibilities.
+    int nchunks = 0, csplit = 1;
+    double *accum = nullptr;
 };
 
 std::vector<int4> tile_pairs(int ntiles);
-hipError_t launch_pair_jones(Kind kind, const PairJonesArgs &a, hipStream_t stream);
+// accum (nullable) backs the component split, which a grid of fewer than about 2048 blocks
+// takes (few tiles outside beam class 0); without it every block runs all components.
+hipError_t launch_pair_jones(Kind kind, const PairJonesArgs &a, DevBuf *accum, hipStream_t stream);
 hipError_t launch_direct_rect(Kind kind, const RectArgs &a, hipStream_t stream);
 hipError_t launch_direct_tiled(Kind kind, const TiledArgs &a, hipStream_t stream);
 hipError_t launch_shapelet_chan(bool per_tile, const ShapeletChanArgs &a, hipStream_t stream);

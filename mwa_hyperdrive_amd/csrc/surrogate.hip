@@ -2,7 +2,7 @@
 // BeamGpu::calc_jones_pair, src/beam/mod.rs:143-177).
 //
 // Network: 53 inputs -> 4 hidden layers of 256 with SiLU -> 8 outputs (re, im of J00, J01,
-// J10, J11). Inputs: sin az, cos az, sin za, cos za, (f - 182 MHz) / 15 MHz, the 16 dipole
+// J10, J11); other widths and depths train and evaluate in FP32 (sm_surrogate_new_shaped). Inputs: sin az, cos az, sin za, cos za, (f - 182 MHz) / 15 MHz, the 16 dipole
 // delays / 31 and the 32 dipole gains.
 //
 // Training runs on the device with the kernels below: GEMMs on MFMA in FP32
@@ -27,6 +27,7 @@ constexpr int kNOut = 8;
 constexpr int kLayers = 5;
 constexpr int kDimIn[kLayers] = {kNIn, kHid, kHid, kHid, kHid};
 constexpr int kDimOut[kLayers] = {kHid, kHid, kHid, kHid, kNOut};
+constexpr int kMaxLayers = 9;  // up to 8 hidden layers in the FP32 path
 
 typedef float f16v __attribute__((ext_vector_type(16)));
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
@@ -518,11 +519,14 @@ __global__ __launch_bounds__(kMW * 64) void k_mlp16(int n, const float *__restri
 }
 
 struct SmSurrogate {
-    DevBuf w[kLayers], b[kLayers], mw[kLayers], vw[kLayers], mb[kLayers], vb[kLayers], gw[kLayers], gb[kLayers];
+    int hid = kHid, nl = kLayers;          // hidden width; layers (hidden layers + output)
+    int din[kMaxLayers] = {}, dout[kMaxLayers] = {};
+    DevBuf w[kMaxLayers], b[kMaxLayers], mw[kMaxLayers], vw[kMaxLayers], mb[kMaxLayers], vb[kMaxLayers],
+        gw[kMaxLayers], gb[kMaxLayers];
     DevBuf w16[kLayers];
     DevBuf w0p, w4p;  // FP16 layer 0 padded to K = 64 and layer 4 padded to 32 outputs
     bool w16_valid = false;
-    DevBuf z[kLayers], h[kLayers];  // training activations: z pre-activation, h post (h[0] = input)
+    DevBuf z[kMaxLayers], h[kMaxLayers];  // training activations: z pre-activation, h post (h[0] = input)
     DevBuf dz, dh, dz2, sse;
     DevBuf a16[2], zf;              // inference ping-pong FP16 activations and FP32 scratch
     DevBuf bits, feat, fout;
@@ -531,13 +535,17 @@ struct SmSurrogate {
 
 namespace {
 
-size_t wsize(int l) { return (size_t)kDimOut[l] * kDimIn[l]; }
+size_t wsize(const SmSurrogate *s, int l) { return (size_t)s->dout[l] * s->din[l]; }
+
+// The shape the fused FP16 inference kernel is built for.
+bool fused_shape(const SmSurrogate *s) { return s->hid == kHid && s->nl == kLayers; }
 
 const char *ensure_w16(SmSurrogate *s) {
+    if (!fused_shape(s)) return "the FP16 network is 4 x 256; other shapes evaluate in FP32 (dtype 0)";
     if (s->w16_valid) return nullptr;
     for (int l = 0; l < kLayers; ++l) {
-        SM_TRY(s->w16[l].reserve(sizeof(_Float16) * wsize(l)));
-        hipLaunchKernelGGL(k_to_half, dim3(blocks(wsize(l))), dim3(256), 0, 0, wsize(l), s->w[l].as<float>(),
+        SM_TRY(s->w16[l].reserve(sizeof(_Float16) * wsize(s, l)));
+        hipLaunchKernelGGL(k_to_half, dim3(blocks(wsize(s, l))), dim3(256), 0, 0, wsize(s, l), s->w[l].as<float>(),
                            s->w16[l].as<_Float16>());
     }
     SM_TRY(s->w0p.reserve(sizeof(_Float16) * kHid * kK0));
@@ -551,17 +559,18 @@ const char *ensure_w16(SmSurrogate *s) {
     return nullptr;
 }
 
-// FP32 forward over n rows, keeping z and h of every layer; output in z[4].
+// FP32 forward over n rows, keeping z and h of every layer; output in z[nl - 1].
 const char *forward32(SmSurrogate *s, int n, const float *d_x) {
     const float *hin = d_x;
-    for (int l = 0; l < kLayers; ++l) {
-        SM_TRY(s->z[l].reserve(sizeof(float) * (size_t)n * kDimOut[l]));
-        if (l < kLayers - 1) SM_TRY(s->h[l + 1].reserve(sizeof(float) * (size_t)n * kDimOut[l]));
-        const int epi = l < kLayers - 1 ? EPI_BIAS_SILU : EPI_BIAS;
-        SM_TRY((gemm<float, float, false, true>(n, kDimOut[l], kDimIn[l], hin, kDimIn[l], s->w[l].as<float>(),
-                                                kDimIn[l], s->z[l].as<float>(), kDimOut[l], epi, s->b[l].as<float>(),
-                                                l < kLayers - 1 ? s->h[l + 1].ptr : nullptr, 0)));
-        if (l < kLayers - 1) hin = s->h[l + 1].as<float>();
+    const int L = s->nl;
+    for (int l = 0; l < L; ++l) {
+        SM_TRY(s->z[l].reserve(sizeof(float) * (size_t)n * s->dout[l]));
+        if (l < L - 1) SM_TRY(s->h[l + 1].reserve(sizeof(float) * (size_t)n * s->dout[l]));
+        const int epi = l < L - 1 ? EPI_BIAS_SILU : EPI_BIAS;
+        SM_TRY((gemm<float, float, false, true>(n, s->dout[l], s->din[l], hin, s->din[l], s->w[l].as<float>(),
+                                                s->din[l], s->z[l].as<float>(), s->dout[l], epi, s->b[l].as<float>(),
+                                                l < L - 1 ? s->h[l + 1].ptr : nullptr, 0)));
+        if (l < L - 1) hin = s->h[l + 1].as<float>();
     }
     return nullptr;
 }
@@ -570,28 +579,29 @@ const char *forward32(SmSurrogate *s, int n, const float *d_x) {
 const char *backward32(SmSurrogate *s, int n, const float *d_x, const float *d_y, double *loss) {
     if (const char *e = forward32(s, n, d_x)) return e;
     SM_TRY(s->sse.reserve(sizeof(double)));
-    SM_TRY(s->dz.reserve(sizeof(float) * (size_t)n * kHid));
-    SM_TRY(s->dh.reserve(sizeof(float) * (size_t)n * kHid));
-    SM_TRY(s->dz2.reserve(sizeof(float) * (size_t)n * kHid));
+    const int wmax = s->hid > kNOut ? s->hid : kNOut;
+    SM_TRY(s->dz.reserve(sizeof(float) * (size_t)n * wmax));
+    SM_TRY(s->dh.reserve(sizeof(float) * (size_t)n * wmax));
+    SM_TRY(s->dz2.reserve(sizeof(float) * (size_t)n * wmax));
     hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(1), 0, 0, s->sse.as<double>());
     const size_t no = (size_t)n * kNOut;
-    hipLaunchKernelGGL(k_mse, dim3(blocks(no)), dim3(256), 0, 0, no, s->z[kLayers - 1].as<float>(), d_y,
+    hipLaunchKernelGGL(k_mse, dim3(blocks(no)), dim3(256), 0, 0, no, s->z[s->nl - 1].as<float>(), d_y,
                        s->dz.as<float>(), s->sse.as<double>());
     float *dz = s->dz.as<float>(), *dzn = s->dz2.as<float>();
-    for (int l = kLayers - 1; l >= 0; --l) {
+    for (int l = s->nl - 1; l >= 0; --l) {
         const float *hin = l == 0 ? d_x : s->h[l].as<float>();
         // dW = dZ^T H_in (out x in), db = column sums of dZ.
-        SM_TRY((gemm<float, float, true, false>(kDimOut[l], kDimIn[l], n, dz, kDimOut[l], hin, kDimIn[l],
-                                                s->gw[l].as<float>(), kDimIn[l], EPI_STORE, nullptr, nullptr, 0)));
-        hipLaunchKernelGGL(k_zero_f, dim3(1), dim3(256), 0, 0, kDimOut[l], s->gb[l].as<float>());
-        hipLaunchKernelGGL(k_colsum, dim3((kDimOut[l] + 63) / 64, (n + kColRows - 1) / kColRows), dim3(256), 0, 0, n,
-                           kDimOut[l], dz, s->gb[l].as<float>());
+        SM_TRY((gemm<float, float, true, false>(s->dout[l], s->din[l], n, dz, s->dout[l], hin, s->din[l],
+                                                s->gw[l].as<float>(), s->din[l], EPI_STORE, nullptr, nullptr, 0)));
+        hipLaunchKernelGGL(k_zero_f, dim3(1), dim3(256), 0, 0, s->dout[l], s->gb[l].as<float>());
+        hipLaunchKernelGGL(k_colsum, dim3((s->dout[l] + 63) / 64, (n + kColRows - 1) / kColRows), dim3(256), 0, 0, n,
+                           s->dout[l], dz, s->gb[l].as<float>());
         if (l == 0) break;
         // dH_in = dZ W (n x in), then through SiLU of the layer below.
-        SM_TRY((gemm<float, float, false, false>(n, kDimIn[l], kDimOut[l], dz, kDimOut[l], s->w[l].as<float>(),
-                                                 kDimIn[l], s->dh.as<float>(), kDimIn[l], EPI_STORE, nullptr, nullptr,
+        SM_TRY((gemm<float, float, false, false>(n, s->din[l], s->dout[l], dz, s->dout[l], s->w[l].as<float>(),
+                                                 s->din[l], s->dh.as<float>(), s->din[l], EPI_STORE, nullptr, nullptr,
                                                  0)));
-        const size_t nh = (size_t)n * kDimIn[l];
+        const size_t nh = (size_t)n * s->din[l];
         hipLaunchKernelGGL(k_silu_back, dim3(blocks(nh)), dim3(256), 0, 0, nh, s->dh.as<float>(),
                            s->z[l - 1].as<float>(), dzn);
         std::swap(dz, dzn);
@@ -625,14 +635,22 @@ extern "C" {
 
 int32_t sm_surrogate_num_params(void) {
     size_t n = 0;
-    for (int l = 0; l < kLayers; ++l) n += wsize(l) + kDimOut[l];
+    for (int l = 0; l < kLayers; ++l) n += (size_t)kDimOut[l] * kDimIn[l] + kDimOut[l];
     return (int32_t)n;
 }
 
-const char *sm_surrogate_new(uint64_t seed, SmSurrogate **out) {
+const char *sm_surrogate_new_shaped(uint64_t seed, int32_t hidden, int32_t depth, SmSurrogate **out) {
     if (out == nullptr) return "null argument";
     *out = nullptr;
+    if (hidden < 1 || hidden > 4096) return "hidden width must be 1..4096";
+    if (depth < 1 || depth > kMaxLayers - 1) return "depth must be 1..8 hidden layers";
     SmSurrogate *s = new SmSurrogate();
+    s->hid = hidden;
+    s->nl = depth + 1;
+    for (int l = 0; l < s->nl; ++l) {
+        s->din[l] = l == 0 ? kNIn : hidden;
+        s->dout[l] = l == s->nl - 1 ? kNOut : hidden;
+    }
     // Uniform(-sqrt(6 / fan_in), +) weights (He-style for SiLU), zero biases, from a host
     // SplitMix64 stream.
     unsigned long long st = seed;
@@ -643,9 +661,9 @@ const char *sm_surrogate_new(uint64_t seed, SmSurrogate **out) {
         z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
         return (double)((z ^ (z >> 31)) >> 11) * 0x1.0p-53;
     };
-    for (int l = 0; l < kLayers; ++l) {
-        std::vector<float> w(wsize(l)), zb(kDimOut[l], 0.0f), zw(wsize(l), 0.0f);
-        const double a = std::sqrt(6.0 / kDimIn[l]) * (l == kLayers - 1 ? 0.25 : 1.0);
+    for (int l = 0; l < s->nl; ++l) {
+        std::vector<float> w(wsize(s, l)), zb(s->dout[l], 0.0f), zw(wsize(s, l), 0.0f);
+        const double a = std::sqrt(6.0 / s->din[l]) * (l == s->nl - 1 ? 0.25 : 1.0);
         for (auto &x : w) x = (float)((2.0 * next() - 1.0) * a);
         const char *e = nullptr;
         auto put = [&](DevBuf &d, const std::vector<float> &v) {
@@ -671,6 +689,18 @@ const char *sm_surrogate_new(uint64_t seed, SmSurrogate **out) {
     return nullptr;
 }
 
+const char *sm_surrogate_new(uint64_t seed, SmSurrogate **out) { return sm_surrogate_new_shaped(seed, kHid, 4, out); }
+
+const char *sm_surrogate_shape(const SmSurrogate *s, int32_t *hidden, int32_t *depth, int64_t *num_params) {
+    if (s == nullptr) return "null argument";
+    size_t n = 0;
+    for (int l = 0; l < s->nl; ++l) n += wsize(s, l) + s->dout[l];
+    if (hidden != nullptr) *hidden = s->hid;
+    if (depth != nullptr) *depth = s->nl - 1;
+    if (num_params != nullptr) *num_params = (int64_t)n;
+    return nullptr;
+}
+
 void sm_surrogate_free(SmSurrogate *s) {
     if (s == nullptr) return;
     (void)hipDeviceSynchronize();
@@ -680,11 +710,11 @@ void sm_surrogate_free(SmSurrogate *s) {
 const char *sm_surrogate_get_params(SmSurrogate *s, float *host) {
     if (s == nullptr || host == nullptr) return "null argument";
     size_t o = 0;
-    for (int l = 0; l < kLayers; ++l) {
-        SM_TRY(hipMemcpy(host + o, s->w[l].ptr, sizeof(float) * wsize(l), hipMemcpyDeviceToHost));
-        o += wsize(l);
-        SM_TRY(hipMemcpy(host + o, s->b[l].ptr, sizeof(float) * kDimOut[l], hipMemcpyDeviceToHost));
-        o += kDimOut[l];
+    for (int l = 0; l < s->nl; ++l) {
+        SM_TRY(hipMemcpy(host + o, s->w[l].ptr, sizeof(float) * wsize(s, l), hipMemcpyDeviceToHost));
+        o += wsize(s, l);
+        SM_TRY(hipMemcpy(host + o, s->b[l].ptr, sizeof(float) * s->dout[l], hipMemcpyDeviceToHost));
+        o += s->dout[l];
     }
     return nullptr;
 }
@@ -692,11 +722,11 @@ const char *sm_surrogate_get_params(SmSurrogate *s, float *host) {
 const char *sm_surrogate_set_params(SmSurrogate *s, const float *host) {
     if (s == nullptr || host == nullptr) return "null argument";
     size_t o = 0;
-    for (int l = 0; l < kLayers; ++l) {
-        SM_TRY(hipMemcpy(s->w[l].ptr, host + o, sizeof(float) * wsize(l), hipMemcpyHostToDevice));
-        o += wsize(l);
-        SM_TRY(hipMemcpy(s->b[l].ptr, host + o, sizeof(float) * kDimOut[l], hipMemcpyHostToDevice));
-        o += kDimOut[l];
+    for (int l = 0; l < s->nl; ++l) {
+        SM_TRY(hipMemcpy(s->w[l].ptr, host + o, sizeof(float) * wsize(s, l), hipMemcpyHostToDevice));
+        o += wsize(s, l);
+        SM_TRY(hipMemcpy(s->b[l].ptr, host + o, sizeof(float) * s->dout[l], hipMemcpyHostToDevice));
+        o += s->dout[l];
     }
     s->w16_valid = false;
     return nullptr;
@@ -720,11 +750,11 @@ const char *sm_surrogate_gradients(SmSurrogate *s, int32_t n, const float *d_x, 
     if (const char *e = backward32(s, n, d_x, d_y, loss)) return e;
     if (host_grads != nullptr) {
         size_t o = 0;
-        for (int l = 0; l < kLayers; ++l) {
-            SM_TRY(hipMemcpy(host_grads + o, s->gw[l].ptr, sizeof(float) * wsize(l), hipMemcpyDeviceToHost));
-            o += wsize(l);
-            SM_TRY(hipMemcpy(host_grads + o, s->gb[l].ptr, sizeof(float) * kDimOut[l], hipMemcpyDeviceToHost));
-            o += kDimOut[l];
+        for (int l = 0; l < s->nl; ++l) {
+            SM_TRY(hipMemcpy(host_grads + o, s->gw[l].ptr, sizeof(float) * wsize(s, l), hipMemcpyDeviceToHost));
+            o += wsize(s, l);
+            SM_TRY(hipMemcpy(host_grads + o, s->gb[l].ptr, sizeof(float) * s->dout[l], hipMemcpyDeviceToHost));
+            o += s->dout[l];
         }
     }
     return nullptr;
@@ -738,10 +768,10 @@ const char *sm_surrogate_train_step(SmSurrogate *s, int32_t n, const float *d_x,
     s->t += 1;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     const float c1 = 1.0f - std::pow(b1, (float)s->t), c2 = 1.0f - std::pow(b2, (float)s->t);
-    for (int l = 0; l < kLayers; ++l) {
-        hipLaunchKernelGGL(k_adam, dim3(blocks(wsize(l))), dim3(256), 0, 0, wsize(l), s->w[l].as<float>(),
+    for (int l = 0; l < s->nl; ++l) {
+        hipLaunchKernelGGL(k_adam, dim3(blocks(wsize(s, l))), dim3(256), 0, 0, wsize(s, l), s->w[l].as<float>(),
                            s->gw[l].as<float>(), s->mw[l].as<float>(), s->vw[l].as<float>(), lr, b1, b2, eps, c1, c2);
-        hipLaunchKernelGGL(k_adam, dim3(blocks(kDimOut[l])), dim3(256), 0, 0, (size_t)kDimOut[l], s->b[l].as<float>(),
+        hipLaunchKernelGGL(k_adam, dim3(blocks(s->dout[l])), dim3(256), 0, 0, (size_t)s->dout[l], s->b[l].as<float>(),
                            s->gb[l].as<float>(), s->mb[l].as<float>(), s->vb[l].as<float>(), lr, b1, b2, eps, c1, c2);
     }
     SM_TRY(hipGetLastError());
@@ -754,7 +784,7 @@ const char *sm_surrogate_forward(SmSurrogate *s, int32_t n, const float *d_x, in
     if (n <= 0) return nullptr;
     if (dtype == 0) {
         if (const char *e = forward32(s, n, d_x)) return e;
-        SM_TRY(hipMemcpyAsync(d_y, s->z[kLayers - 1].ptr, sizeof(float) * (size_t)n * kNOut, hipMemcpyDeviceToDevice,
+        SM_TRY(hipMemcpyAsync(d_y, s->z[s->nl - 1].ptr, sizeof(float) * (size_t)n * kNOut, hipMemcpyDeviceToDevice,
                               0));
         return nullptr;
     }
@@ -785,7 +815,13 @@ const char *sm_surrogate_beam(SmSurrogate *s, int32_t n_tiles, const uint32_t *d
         const double *gn = dg.as<double>() + 32 * (size_t)t;
         hipLaunchKernelGGL(k_tile_features, dim3(blocks(per_tile)), dim3(256), 0, 0, n_freqs, df.as<double>(), n_dirs,
                            d_az, d_za, dl, gn, s->feat.as<float>());
-        if (const char *e = forward16(s, (int)per_tile, s->feat.as<float>(), s->fout.as<float>())) return e;
+        if (fused_shape(s)) {
+            if (const char *e = forward16(s, (int)per_tile, s->feat.as<float>(), s->fout.as<float>())) return e;
+        } else {
+            if (const char *e = forward32(s, (int)per_tile, s->feat.as<float>())) return e;
+            SM_TRY(hipMemcpyAsync(s->fout.ptr, s->z[s->nl - 1].ptr, sizeof(float) * per_tile * kNOut,
+                                  hipMemcpyDeviceToDevice, 0));
+        }
         hipLaunchKernelGGL(k_out_to_jones, dim3(blocks(per_tile)), dim3(256), 0, 0, per_tile, s->fout.as<float>(), out);
         // Judge: the exact beam on the probe directions; the whole tile takes the exact beam
         // where the surrogate misses by more than the threshold.

@@ -1,11 +1,12 @@
 """Python mirror of hyperdrive's SkyModeller trait over the HIP library.
 
 `new_sky_modeller` / `SkyModeller.model_timestep(_with)` / `update_with_a_source` /
-`model_timestep_autos_with` follow src/model/mod.rs:123-261 with the Epoch replaced by
-(LST, latitude): precession and LMST are the caller's (src/model/cpu.rs:731-769 call
-marlu for them). Visibilities are [freq][baseline] Jones, stored as float32/float64
-arrays of shape (num_freqs, num_baselines, 8) = [XX re, XX im, XY re, XY im, YX re,
-YX im, YY re, YY im].
+`model_timestep_autos_with` follow src/model/mod.rs:123-261. The timestep methods take an
+Epoch, as the reference's do: the modeller forms the LST, UVWs and latitude with the DUT1 and
+precession setting it was made with (get_lst_uvws_latitude, src/model/cpu.rs:731-769). They
+also take (LST, latitude) directly in place of the Epoch, for callers that form those
+themselves. Visibilities are [freq][baseline] Jones, stored as float32/float64 arrays of shape
+(num_freqs, num_baselines, 8) = [XX re, XX im, XY re, XY im, YX re, YX im, YY re, YY im].
 """
 
 from __future__ import annotations
@@ -17,6 +18,7 @@ from typing import Optional, Sequence
 import numpy as np
 
 from . import _lib
+from .precession import Epoch
 from .srclist import Source, SourceList, pack_source_list
 
 POLS = {"XX XY YX YY": 0, "XX": 1, "YY": 2, "XX YY": 3, "XX YY XY": 4}
@@ -60,7 +62,8 @@ def analytic_beam_device(delays, gains, freqs, d_az, d_za):
 
 class SkyModeller:
     def __init__(self, beam, source_list, pols, unflagged_tile_xyzs, unflagged_fine_chan_freqs, flagged_tiles,
-                 phase_centre, array_longitude_rad, array_latitude_rad):
+                 phase_centre, array_longitude_rad, array_latitude_rad, dut1: float = 0.0,
+                 apply_precession: bool = False):
         L = _lib.lib()
         self._xyzs = np.ascontiguousarray(np.asarray(unflagged_tile_xyzs, np.float64).reshape(-1, 3))
         self._freqs = np.ascontiguousarray(np.asarray(unflagged_fine_chan_freqs, np.float64))
@@ -80,6 +83,8 @@ class SkyModeller:
         p.phase_centre_ra, p.phase_centre_dec = phase_centre
         p.array_longitude = array_longitude_rad
         p.array_latitude = array_latitude_rad
+        p.dut1 = float(dut1)
+        p.apply_precession = 1 if apply_precession else 0
         p.pols = POLS[pols] if isinstance(pols, str) else int(pols)
         self._delays = self._gains = None
         if isinstance(beam, AnalyticBeam):
@@ -111,28 +116,65 @@ class SkyModeller:
             self._h = None
 
     # ---------------------------------------------------------------- trait surface
-    def model_timestep(self, lst: float, latitude: float, want_f64: bool = False):
-        vis = np.zeros((self.num_freqs, self.num_baselines, 8), np.float32)
-        vis64 = np.zeros((self.num_freqs, self.num_baselines, 8), np.float64) if want_f64 else None
-        uvws = self.model_timestep_with(lst, latitude, vis, vis64)
-        return (vis, uvws, vis64) if want_f64 else (vis, uvws)
-
-    def model_timestep_with(self, lst: float, latitude: float, vis_fb: np.ndarray,
-                            vis_fb_f64: Optional[np.ndarray] = None) -> np.ndarray:
-        """Accumulates into vis_fb (float32 [freq][baseline][8]); returns the baseline UVWs."""
+    def _shape_check(self, vis_fb, vis_fb_f64):
         assert vis_fb.dtype == np.float32 and vis_fb.flags.c_contiguous
         assert vis_fb.shape == (self.num_freqs, self.num_baselines, 8)
         if vis_fb_f64 is not None:
             assert vis_fb_f64.dtype == np.float64 and vis_fb_f64.shape == vis_fb.shape
+
+    def model_timestep(self, timestamp, latitude: Optional[float] = None, want_f64: bool = False):
+        """Visibilities and UVWs of one timestep: `timestamp` is an Epoch, or an LST with
+        `latitude`."""
+        vis = np.zeros((self.num_freqs, self.num_baselines, 8), np.float32)
+        vis64 = np.zeros((self.num_freqs, self.num_baselines, 8), np.float64) if want_f64 else None
+        if isinstance(timestamp, Epoch):
+            if latitude is not None:
+                raise TypeError("an Epoch carries its own LST and latitude")
+            uvws = self.model_timestep_with(timestamp, vis, vis64)
+        else:
+            uvws = self.model_timestep_with(timestamp, latitude, vis, vis64)
+        return (vis, uvws, vis64) if want_f64 else (vis, uvws)
+
+    def model_timestep_with(self, timestamp, *args) -> np.ndarray:
+        """Accumulates into vis_fb (float32 [freq][baseline][8]); returns the baseline UVWs.
+        Arguments: (epoch, vis_fb[, vis_fb_f64]) or (lst, latitude, vis_fb[, vis_fb_f64])."""
         uvws = np.zeros((self.num_baselines, 3))
-        _lib.call("sm_model_timestep_with", self._h, lst, latitude, _ptr(uvws), _ptr(vis_fb), _ptr(vis_fb_f64))
+        if isinstance(timestamp, Epoch):
+            vis_fb, vis_fb_f64 = (tuple(args) + (None,))[:2]
+            self._shape_check(vis_fb, vis_fb_f64)
+            _lib.call("sm_model_timestep_epoch_with", self._h, timestamp.to_gpst_seconds(), _ptr(uvws), _ptr(vis_fb),
+                      _ptr(vis_fb_f64))
+        else:
+            latitude, vis_fb, vis_fb_f64 = (tuple(args) + (None,))[:3]
+            self._shape_check(vis_fb, vis_fb_f64)
+            _lib.call("sm_model_timestep_with", self._h, timestamp, latitude, _ptr(uvws), _ptr(vis_fb),
+                      _ptr(vis_fb_f64))
         return uvws
 
-    def model_timestep_with_device(self, lst: float, latitude: float, d_vis_fb, d_vis_fb_f64=None, d_uvws=None):
-        """Device-buffer form (torch tensors on the GPU); enqueues and returns."""
+    def model_timestep_with_device(self, timestamp, *args, d_uvws=None):
+        """Device-buffer form (torch tensors on the GPU); enqueues and returns. Arguments:
+        (epoch, d_vis_fb[, d_vis_fb_f64]) or (lst, latitude, d_vis_fb[, d_vis_fb_f64[, d_uvws]]);
+        d_uvws (LST form only) overrides the UVWs."""
+        if isinstance(timestamp, Epoch):
+            d_vis_fb, d_vis_fb_f64 = (tuple(args) + (None,))[:2]
+            assert d_uvws is None, "the Epoch form computes its own UVWs"
+            assert d_vis_fb.is_cuda and d_vis_fb.numel() == self.num_freqs * self.num_baselines * 8
+            _lib.call("sm_model_timestep_epoch_with_device", self._h, timestamp.to_gpst_seconds(),
+                      _dev_ptr(d_vis_fb), _dev_ptr(d_vis_fb_f64))
+            return
+        latitude, d_vis_fb, d_vis_fb_f64, pos_uvws = (tuple(args) + (None, None))[:4]
+        if d_uvws is None:
+            d_uvws = pos_uvws
         assert d_vis_fb.is_cuda and d_vis_fb.numel() == self.num_freqs * self.num_baselines * 8
-        _lib.call("sm_model_timestep_with_device", self._h, lst, latitude, _dev_ptr(d_uvws), _dev_ptr(d_vis_fb),
+        _lib.call("sm_model_timestep_with_device", self._h, timestamp, latitude, _dev_ptr(d_uvws), _dev_ptr(d_vis_fb),
                   _dev_ptr(d_vis_fb_f64))
+
+    def lst_uvws_latitude(self, epoch: Epoch):
+        """(LST, cross UVWs [baseline][3], latitude) of a timestamp (get_lst_uvws_latitude)."""
+        lst, lat = C.c_double(), C.c_double()
+        uvws = np.zeros((self.num_baselines, 3))
+        _lib.call("sm_lst_uvws_latitude", self._h, epoch.to_gpst_seconds(), C.byref(lst), C.byref(lat), _ptr(uvws))
+        return lst.value, uvws, lat.value
 
     def update_with_a_source(self, source: Source, phase_centre) -> None:
         packed = pack_source_list([source])
@@ -140,11 +182,20 @@ class SkyModeller:
                   phase_centre[0], phase_centre[1])
         self._keep = packed
 
-    def model_timestep_autos_with(self, lst: float, latitude: float, vis_fb: Optional[np.ndarray] = None,
-                                  vis_fb_f64: Optional[np.ndarray] = None):
+    def model_timestep_autos_with(self, timestamp, *args):
+        """Autos [freq][tile][8]: (epoch[, vis_fb[, vis_fb_f64]]) or (lst, latitude[, vis_fb[,
+        vis_fb_f64]])."""
+        if isinstance(timestamp, Epoch):
+            vis_fb, vis_fb_f64 = (tuple(args) + (None, None))[:2]
+        else:
+            latitude, vis_fb, vis_fb_f64 = (tuple(args) + (None, None))[:3]
         if vis_fb is None:
             vis_fb = np.zeros((self.num_freqs, self.num_tiles, 8), np.float32)
-        _lib.call("sm_model_timestep_autos_with", self._h, lst, latitude, _ptr(vis_fb), _ptr(vis_fb_f64))
+        if isinstance(timestamp, Epoch):
+            _lib.call("sm_model_timestep_autos_epoch_with", self._h, timestamp.to_gpst_seconds(), _ptr(vis_fb),
+                      _ptr(vis_fb_f64))
+        else:
+            _lib.call("sm_model_timestep_autos_with", self._h, timestamp, latitude, _ptr(vis_fb), _ptr(vis_fb_f64))
         return vis_fb
 
     def set_beam_surrogate(self, surrogate, threshold: float) -> int:
@@ -167,7 +218,9 @@ class SkyModeller:
 
 
 def new_sky_modeller(beam, source_list, pols, unflagged_tile_xyzs, unflagged_fine_chan_freqs, flagged_tiles,
-                     phase_centre, array_longitude_rad, array_latitude_rad) -> SkyModeller:
-    """new_sky_modeller (src/model/mod.rs:215-261); the device is always the MI355X."""
+                     phase_centre, array_longitude_rad, array_latitude_rad, dut1: float = 0.0,
+                     apply_precession: bool = False) -> SkyModeller:
+    """new_sky_modeller (src/model/mod.rs:215-261); the device is always the MI355X. dut1 in
+    seconds."""
     return SkyModeller(beam, source_list, pols, unflagged_tile_xyzs, unflagged_fine_chan_freqs, flagged_tiles,
-                       phase_centre, array_longitude_rad, array_latitude_rad)
+                       phase_centre, array_longitude_rad, array_latitude_rad, dut1, apply_precession)

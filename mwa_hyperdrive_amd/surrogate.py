@@ -39,12 +39,16 @@ def samples(seed: int, first: int, n: int, want_raw: bool = False, mode: int = P
 
 
 class BeamSurrogate:
-    """53 -> 4 x 256 SiLU -> 8 MLP; parameters live on the device."""
+    """53 -> depth x hidden SiLU -> 8 MLP (4 x 256 by default); parameters live on the device.
+    Shapes other than 4 x 256 train and evaluate in FP32."""
 
-    def __init__(self, seed: int = 1):
+    def __init__(self, seed: int = 1, hidden: int = 256, depth: int = 4):
         h = C.c_void_p()
-        _lib.call("sm_surrogate_new", C.c_uint64(seed), C.byref(h))
+        _lib.call("sm_surrogate_new_shaped", C.c_uint64(seed), hidden, depth, C.byref(h))
         self._h = h
+        n = C.c_int64()
+        _lib.call("sm_surrogate_shape", h, None, None, C.byref(n))
+        self.hidden, self.depth, self._nparams = hidden, depth, int(n.value)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -55,9 +59,8 @@ class BeamSurrogate:
                 pass
             self._h = None
 
-    @staticmethod
-    def num_params() -> int:
-        return int(_lib.lib().sm_surrogate_num_params())
+    def num_params(self) -> int:
+        return self._nparams
 
     def get_params(self) -> np.ndarray:
         p = np.empty(self.num_params(), dtype=np.float32)
@@ -83,6 +86,7 @@ class BeamSurrogate:
         return loss.value
 
     def forward(self, x, fp16: bool = True):
+        """Outputs [n][8]; fp16 (the fused kernel) only for the 4 x 256 network."""
         import torch
 
         y = torch.empty((x.shape[0], N_OUT), dtype=torch.float32, device=x.device)
@@ -133,3 +137,26 @@ class BeamSurrogate:
                   gains.ctypes.data_as(C.c_void_p), nf, freqs.ctypes.data_as(C.c_void_p), nd, _ptr(d_az),
                   _ptr(d_za), float(threshold), _ptr(out), C.byref(nfb))
         return out, nfb.value
+
+
+def accuracy_sweep(runs, n_samples: int = 1 << 21, batch: int = 4096, lr: float = 2e-3, heldout: int = 100_000,
+                   log=None) -> list:
+    """What accuracy a network size and training length reach: for each (hidden, depth,
+    epochs) in `runs`, a network trained from the same seed on the same pointing-delay samples,
+    then its held-out Jones error (FP32 forward, 100 000 samples of another seed). Returns one
+    dict per run: shape, parameters, epochs, training seconds, held-out RMS and max |dJ|."""
+    xh, yh = samples(99, 0, heldout)
+    rows = []
+    for hidden, depth, epochs in runs:
+        s = BeamSurrogate(seed=1, hidden=hidden, depth=depth)
+        tr = s.train(n_samples=n_samples, epochs=epochs, batch=batch, lr=lr, log=log)
+        d = s.forward(xh, fp16=False) - yh
+        row = {"hidden": hidden, "depth": depth, "params": s.num_params(), "epochs": epochs,
+               "train_seconds": tr["seconds"], "heldout_rms": d.pow(2).mean().sqrt().item(),
+               "heldout_max": d.abs().max().item()}
+        rows.append(row)
+        if log is not None:
+            log(f"{depth} x {hidden}, {epochs} epochs: rms {row['heldout_rms']:.3e}, max {row['heldout_max']:.3e} "
+                f"({row['train_seconds']:.1f} s)")
+        del s, d
+    return rows

@@ -36,6 +36,7 @@ class OrcObs(C.Structure):
         ("dipole_delays", C.POINTER(C.c_uint32)),
         ("dipole_gains", C.POINTER(C.c_double)),
         ("pols", C.c_int32),
+        ("uvw_xyzs", C.POINTER(C.c_double)),
     ]
 
 
@@ -77,7 +78,7 @@ class Obs:
     """Observation parameters for the oracle (arguments of SkyModellerCpu::new)."""
 
     def __init__(self, xyzs, freqs, phase_centre, lst, latitude, flagged_tiles=(), beam="none",
-                 delays=None, gains=None, pols=0):
+                 delays=None, gains=None, pols=0, uvw_xyzs=None):
         self.xyzs = np.ascontiguousarray(np.asarray(xyzs, dtype=np.float64).reshape(-1, 3))
         self.freqs = np.ascontiguousarray(np.asarray(freqs, dtype=np.float64))
         self.flagged = np.ascontiguousarray(np.asarray(sorted(flagged_tiles), dtype=np.int32))
@@ -89,6 +90,9 @@ class Obs:
         self.delays = None if delays is None else np.ascontiguousarray(np.asarray(delays, np.uint32).reshape(total, 16))
         self.gains = None if gains is None else np.ascontiguousarray(np.asarray(gains, np.float64).reshape(total, 32))
         self.pols = pols
+        # Tile XYZs of the cross UVWs when they differ from xyzs (the precessed XYZs).
+        self.uvw_xyzs = None if uvw_xyzs is None else np.ascontiguousarray(
+            np.asarray(uvw_xyzs, np.float64).reshape(self.xyzs.shape))
 
     @property
     def num_baselines(self):
@@ -110,6 +114,7 @@ class Obs:
         o.dipole_delays = self.delays.ctypes.data_as(C.POINTER(C.c_uint32)) if self.delays is not None else None
         o.dipole_gains = self.gains.ctypes.data_as(C.POINTER(C.c_double)) if self.gains is not None else None
         o.pols = self.pols
+        o.uvw_xyzs = self.uvw_xyzs.ctypes.data_as(C.POINTER(C.c_double)) if self.uvw_xyzs is not None else None
         return o
 
 

@@ -472,8 +472,8 @@ static void build_modeller(modeller *m, const OrcObs *obs, const OrcComponent *c
         m->freq_map[f] = idx;
     }
     m->uvws = (double *)malloc(sizeof(double) * 3 * (size_t)(m->num_bl > 0 ? m->num_bl : 1));
-    orc_xyzs_to_cross_uvws(nt, obs->unflagged_xyzs, obs->lst - obs->phase_centre_ra, obs->phase_centre_dec,
-                           m->uvws);
+    orc_xyzs_to_cross_uvws(nt, obs->uvw_xyzs != NULL ? obs->uvw_xyzs : obs->unflagged_xyzs,
+                           obs->lst - obs->phase_centre_ra, obs->phase_centre_dec, m->uvws);
     m->bl_t1 = (int32_t *)malloc(sizeof(int32_t) * (size_t)(m->num_bl > 0 ? m->num_bl : 1));
     m->bl_t2 = (int32_t *)malloc(sizeof(int32_t) * (size_t)(m->num_bl > 0 ? m->num_bl : 1));
     {

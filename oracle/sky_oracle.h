@@ -63,6 +63,10 @@ typedef struct OrcObs {
     const uint32_t *dipole_delays;   /* [total_tiles][16] or NULL (all 0) */
     const double *dipole_gains;      /* [total_tiles][32] or NULL (all 1) */
     int32_t pols;                    /* ORC_POLS_* */
+    /* Tile XYZs the cross UVWs come from, or NULL for unflagged_xyzs. With precession the
+     * reference forms the cross UVWs from the precessed XYZs and the shapelet UVs from the
+     * unprecessed ones (src/model/cpu.rs:741-785). */
+    const double *uvw_xyzs;
 } OrcObs;
 
 /* --- geometry (marlu 0.17.1 restatements; call sites src/model/cpu.rs:767, components/mod.rs:214,394) --- */

@@ -6,7 +6,8 @@ in include/skymodel.h, section "beam surrogate"):
 
 * features: sin az, cos az, sin za, cos za, (f - 182 MHz) / 15 MHz, delays / 31 (a delay of
   32 marks a dead dipole: feature 0 and both its gains 0), 32 gains;
-* network: 53 -> 256 -> 256 -> 256 -> 256 -> 8, SiLU on the hidden layers;
+* network: 53 -> 256 -> 256 -> 256 -> 256 -> 8, SiLU on the hidden layers (other widths and
+  depths through `shape=dims(hidden, depth)`);
 * loss: mean of squared errors over the batch's n x 8 outputs;
 * Adam with beta1 0.9, beta2 0.999, eps 1e-8 and bias correction.
 
@@ -21,18 +22,25 @@ from __future__ import annotations
 import numpy as np
 
 N_IN, N_HID, N_OUT = 53, 256, 8
-DIMS = [(N_HID, N_IN), (N_HID, N_HID), (N_HID, N_HID), (N_HID, N_HID), (N_OUT, N_HID)]
 
 
-def num_params() -> int:
-    return sum(o * i + o for o, i in DIMS)
+def dims(hidden: int = N_HID, depth: int = 4):
+    """(out, in) of each layer: `depth` hidden layers of `hidden` units, then the output."""
+    return [(hidden, N_IN)] + [(hidden, hidden)] * (depth - 1) + [(N_OUT, hidden)]
 
 
-def unpack(flat):
+DIMS = dims()
+
+
+def num_params(shape=DIMS) -> int:
+    return sum(o * i + o for o, i in shape)
+
+
+def unpack(flat, shape=DIMS):
     """Flat parameters (W0, b0, W1, b1, ...) -> list of (W [out][in], b [out])."""
     flat = np.asarray(flat, dtype=np.float64)
     out, o = [], 0
-    for no, ni in DIMS:
+    for no, ni in shape:
         w = flat[o:o + no * ni].reshape(no, ni)
         o += no * ni
         b = flat[o:o + no]
@@ -66,11 +74,11 @@ def _silu(z):
     return z / (1.0 + np.exp(-z))
 
 
-def forward(flat, x):
+def forward(flat, x, shape=DIMS):
     """Returns (y [n][8], cache of (input, pre-activation) per layer)."""
     h = np.asarray(x, dtype=np.float64)
     cache = []
-    layers = unpack(flat)
+    layers = unpack(flat, shape)
     for li, (w, b) in enumerate(layers):
         z = h @ w.T + b
         cache.append((h, z))
@@ -78,14 +86,14 @@ def forward(flat, x):
     return h, cache
 
 
-def loss_and_grads(flat, x, y):
+def loss_and_grads(flat, x, y, shape=DIMS):
     """MSE over all n x 8 outputs and its gradient (flat, same order as the parameters)."""
-    out, cache = forward(flat, x)
+    out, cache = forward(flat, x, shape)
     y = np.asarray(y, dtype=np.float64)
     d = out - y
     loss = float(np.mean(d * d))
     dz = 2.0 * d / d.size
-    layers = unpack(flat)
+    layers = unpack(flat, shape)
     grads = [None] * len(layers)
     for li in range(len(layers) - 1, -1, -1):
         h, _ = cache[li]

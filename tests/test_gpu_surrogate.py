@@ -83,6 +83,26 @@ def test_forward_fp32_and_fp16_vs_oracle():
     assert e16 <= 1e-2
 
 
+@pytest.mark.parametrize("hidden,depth", [(96, 2), (320, 6)])
+def test_shaped_network_vs_oracle(hidden, depth):
+    """Other widths and depths: parameter count, gradients and the FP32 forward against the
+    oracle over the same shape; the fused FP16 forward is for 4 x 256 only."""
+    s = sg.BeamSurrogate(seed=12, hidden=hidden, depth=depth)
+    shape = so.dims(hidden, depth)
+    assert s.num_params() == so.num_params(shape)
+    p = s.get_params().astype(np.float64)
+    x, y = sg.samples(13, 0, 384)
+    loss, g = s.gradients(x, y)
+    ref_loss, ref_g = so.loss_and_grads(p, x.cpu().numpy(), y.cpu().numpy(), shape)
+    assert abs(loss - ref_loss) <= 1e-5 * ref_loss
+    assert np.abs(g - ref_g).max() <= 1e-4 * np.abs(ref_g).max()
+    ref, _ = so.forward(p, x.cpu().numpy(), shape)
+    y32 = s.forward(x, fp16=False).cpu().numpy()
+    assert np.abs(y32 - ref).max() <= 1e-5 * np.abs(ref).max()
+    with pytest.raises(Exception, match="4 x 256"):
+        s.forward(x, fp16=True)
+
+
 def test_training_reduces_error_and_fallback_is_exact():
     s = sg.BeamSurrogate(seed=1)
     xh, yh = sg.samples(99, 0, 20000)  # held out: another seed

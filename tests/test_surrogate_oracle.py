@@ -75,3 +75,22 @@ def test_features_definition():
     assert x[1, 5 + 5] == 0.0 and x[1, 5 + 4] == 1.0
     assert x[0, 21 + 7] == 0.0 and x[1, 21 + 5] == 0.0 and x[1, 21 + 16 + 5] == 0.0
     assert x[1, 21 + 4] == 1.0
+
+
+def test_other_shapes_gradients_match_finite_differences():
+    """The same code over other widths and depths (the shaped networks' checker)."""
+    rng = np.random.default_rng(3)
+    for hidden, depth in ((12, 1), (9, 3)):
+        shape = so.dims(hidden, depth)
+        assert len(shape) == depth + 1 and shape[0] == (hidden, so.N_IN) and shape[-1] == (so.N_OUT, hidden)
+        p = rng.normal(0.0, 0.3, so.num_params(shape))
+        x = rng.normal(size=(5, so.N_IN))
+        y = rng.normal(size=(5, so.N_OUT))
+        _, g = so.loss_and_grads(p, x, y, shape)
+        h = 1e-6
+        for k in rng.choice(p.size, 25, replace=False):
+            pp, pm = p.copy(), p.copy()
+            pp[k] += h
+            pm[k] -= h
+            fd = (so.loss_and_grads(pp, x, y, shape)[0] - so.loss_and_grads(pm, x, y, shape)[0]) / (2 * h)
+            assert abs(fd - g[k]) <= 1e-6 * max(1.0, abs(fd)) + 1e-9, (hidden, depth, k)

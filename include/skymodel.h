@@ -60,6 +60,17 @@ const char *sm_model_gaussians_f64(const Gaussians *comps, const Addresses *a, c
 const char *sm_model_shapelets_f64(const Shapelets *comps, const Addresses *a, const UVW *d_uvws,
                                    const JonesF64 *d_beam_jones, JonesF32 *d_vis_fb, JonesF64 *d_vis_fb_f64);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving or peel loops, harnesses that forbid hipMalloc): the same
+ * inputs, outputs and results as sm_model_points_f64, bit for bit, with every work buffer
+ * (padded frequencies, tile UVWs, tile-group pairs, beam rows, LMN and the coherency table)
+ * taken from one fixed 64 MiB device array per GPU instead of hipMalloc. It calls no
+ * allocating routine. A call whose work buffers exceed the array (about 64 B per component
+ * and padded channel, plus 8 B per baseline with per-tile beams) returns an error. Calls on one
+ * GPU share the array: calls that overlap on the same GPU (other threads, other streams) race. */
+const char *sm_model_points_noalloc(const Points *comps, const Addresses *a, const UVW *d_uvws,
+                                    const JonesF64 *d_beam_jones, JonesF32 *d_vis_fb, JonesF64 *d_vis_fb_f64);
+
 /* ------------------------------------------------------------------ 3. SkyModeller handle */
 
 enum { SM_BEAM_NONE = 0, SM_BEAM_ANALYTIC = 1 };

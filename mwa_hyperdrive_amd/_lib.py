@@ -161,6 +161,7 @@ SIGNATURES = {
     "sm_model_points_f64": [_P, _P, _P, _P, _P, _P],
     "sm_model_gaussians_f64": [_P, _P, _P, _P, _P, _P],
     "sm_model_shapelets_f64": [_P, _P, _P, _P, _P, _P],
+    "sm_model_points_noalloc": [_P, _P, _P, _P, _P, _P],
     "sm_new": [C.POINTER(SmObsParams), _P, _P, _I, C.POINTER(_P)],
     "sm_update_with_a_source": [_P, _P, _I, _D, _D],
     "sm_model_timestep_with": [_P, _D, _D, _P, _P, _P],

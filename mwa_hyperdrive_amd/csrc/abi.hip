@@ -2,6 +2,7 @@
 // shared per-component-type driver used by both these and the SkyModeller handle.
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "sm_abi_common.h"
@@ -499,7 +500,7 @@ const char *gather_shape(Kind kind, int n_pl, const GaussianParams *pl, int n_cp
 
 using namespace sm;
 
-// The body of sm_model_points_f64 over the work buffers in st.
+// The body of sm_model_points_f64 / sm_model_points_noalloc over the work buffers in st.
 static const char *model_points_with(AbiState &st, const Points *p, const Addresses *a, const UVW *d_uvws,
                                      const JonesF64 *d_beam_jones, JonesF32 *d_vis_fb, JonesF64 *d_vis_fb_f64) {
     const int n = p->num_power_laws + p->num_curved_power_laws + p->num_lists;
@@ -524,6 +525,88 @@ extern "C" const char *sm_model_points_f64(const Points *p, const Addresses *a, 
     if (p == nullptr || a == nullptr) return "null argument";
     if (p->num_power_laws + p->num_curved_power_laws + p->num_lists == 0) return nullptr;
     return model_points_with(abi_state(), p, a, d_uvws, d_beam_jones, d_vis_fb, d_vis_fb_f64);
+}
+
+// ------------------------------------------------------------------ allocation-free entry
+// Work memory of sm_model_points_noalloc: one array per GPU, held until the process ends.
+// Host code never passes `scratch` itself (its host-side name is not a device address):
+// noalloc_scratch() reads the device address with hipGetSymbolAddress once per device.
+static __device__ __align__(256) unsigned char scratch[SM_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's copy of `scratch`, looked up on the first call on that device.
+const char *noalloc_scratch(void **out) {
+    int dev = 0;
+    SM_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return "device index beyond the supported range";
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void *ptr = nullptr;
+        SM_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (ptr == nullptr) return "hipGetSymbolAddress returned no address for the scratch array";
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return nullptr;
+}
+
+size_t round_up256(size_t b) { return ((b < 256 ? 256 : b) + 255) / 256 * 256; }
+
+}  // namespace
+
+extern "C" const char *sm_model_points_noalloc(const Points *p, const Addresses *a, const UVW *d_uvws,
+                                               const JonesF64 *d_beam_jones, JonesF32 *d_vis_fb,
+                                               JonesF64 *d_vis_fb_f64) {
+    if (p == nullptr || a == nullptr) return "null argument";
+    const int n = p->num_power_laws + p->num_curved_power_laws + p->num_lists;
+    if (n == 0 || a->num_baselines <= 0 || a->num_freqs <= 0) return nullptr;
+    if (n < 0) return "negative component count";
+    const long long nbl = a->num_baselines;
+    const int ntiles = (int)llround((1.0 + std::sqrt(1.0 + 8.0 * (double)nbl)) / 2.0);
+    if ((long long)ntiles * (ntiles - 1) / 2 != nbl) return "num_baselines is not a triangular number";
+    const size_t nchan_pad = (size_t)pad_channels(a->num_freqs);
+    const size_t groups = (size_t)(ntiles + 15) / 16;
+    const size_t npairs = groups * (groups - 1) / 2 + (groups + 1) / 2;
+    // Every buffer the shared body reserves, in the order it reserves them.
+    const size_t want[7] = {
+        sizeof(double) * nchan_pad,                                  // padded frequencies
+        d_uvws != nullptr ? sizeof(double) * 3 * (size_t)ntiles : 0,  // tile UVWs
+        d_uvws != nullptr ? sizeof(int4) * npairs : 0,               // tile-group pairs
+        d_beam_jones != nullptr ? sizeof(int) * (size_t)ntiles : 0,  // beam row per tile
+        d_beam_jones != nullptr ? sizeof(int2) * (size_t)nbl : 0,    // beam rows per baseline
+        sizeof(double4) * (size_t)n,                                 // LMN
+        sizeof(double) * 8 * (size_t)n * nchan_pad,                  // coherency table
+    };
+    size_t total = 0;
+    for (size_t w : want) total += w > 0 ? round_up256(w) : 0;
+    if (total > (size_t)SM_NOALLOC_SCRATCH_BYTES)
+        return "sm_model_points_noalloc: this call needs more work memory than the fixed scratch array "
+               "holds (SM_NOALLOC_SCRATCH_BYTES; about 64 B per component and padded channel)";
+    void *base = nullptr;
+    if (const char *e = noalloc_scratch(&base)) return e;
+    AbiState st;
+    DevBuf *bufs[7] = {&st.freqs_pad, &st.tile_uvw, &st.pairs, &st.tile_rows, &st.bl_rows, &st.lmn, &st.coh};
+    size_t off = 0;
+    for (int i = 0; i < 7; ++i) {
+        if (want[i] == 0) {
+            bufs[i]->borrow(nullptr, 0);
+            continue;
+        }
+        bufs[i]->borrow(static_cast<unsigned char *>(base) + off, round_up256(want[i]));
+        off += round_up256(want[i]);
+    }
+    // Buffers the point path never reserves: borrowed with no capacity, so that any use
+    // fails with an error instead of allocating.
+    for (DevBuf *b : {&st.shape, &st.bl_uv, &st.coeffs, &st.coeff_range, &st.sbf, &st.accum, &st.scratch.accum,
+                      &st.scratch.cls, &st.scratch.prange, &st.scratch.packed, &st.scratch.coh0, &st.scratch.cseg,
+                      &st.scratch.ccls})
+        b->borrow(nullptr, 0);
+    return model_points_with(st, p, a, d_uvws, d_beam_jones, d_vis_fb, d_vis_fb_f64);
 }
 
 extern "C" const char *model_points(const Points *comps, const Addresses *a, const UVW *d_uvws,

@@ -23,6 +23,8 @@
 
 #define SM_NCH 8
 #define SM_VEL_C 299792458.0
+// Size of the fixed work array of sm_model_points_noalloc (bytes, per GPU).
+#define SM_NOALLOC_SCRATCH_BYTES (64u << 20)
 
 namespace sm {
 

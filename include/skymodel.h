@@ -153,9 +153,10 @@ typedef struct SmInfo {
  * V^e = (K (.) C^e) K^H with K_pc = exp(i 2 pi (u_p l + v_p m + w_p (n - 1)) f / c), phases and
  * their range reduction in FP64, operands split into hi + lo halves of `dtype`, products on
  * MFMA with FP32 accumulation promoted to FP64 every 1024 components. Gaussians and
- * shapelets take the FP64 direct kernels. With per-tile beams, every baseline with a tile of
- * the most common beam row takes the MFMA path (row coherency J_r S J_0^H), and pairs of two
- * tiles of other rows take the FP64 direct kernel. */
+ * shapelets run in FP32 with their phases, phase reductions, exponents and table positions in
+ * FP64. With per-tile beams, every baseline with a tile of the most common beam row takes the
+ * MFMA path (row coherency J_r S J_0^H), and the points of pairs of two tiles of other rows
+ * the own-beam pair kernel in FP32 with FP64 phases. */
 enum { SM_MFMA_FP16 = 0, SM_MFMA_BF16 = 1 };
 const char *sm_model_timestep_factored(SmModeller *m, double lst, double latitude, int32_t dtype,
                                        JonesF32 *d_vis_fb, JonesF64 *d_vis_fb_f64);
@@ -241,6 +242,40 @@ const char *sm_model_timestep_epoch_with_device(SmModeller *m, double gps_second
 /* SkyModeller::model_timestep_autos_with(Epoch) (src/model/mod.rs:195-203). */
 const char *sm_model_timestep_autos_epoch_with(SmModeller *m, double gps_seconds, JonesF32 *vis_fb,
                                                JonesF64 *vis_fb_f64);
+
+/* ------------------------------------------------------------------ uvfits output
+ * The vis-simulate writer path (src/io/write/mod.rs:113-260, src/params/vis_simulate.rs:241-300):
+ * modelled visibilities written as AIPS random-groups uvfits, as marlu 0.17.1's UvfitsWriter
+ * writes them for the reference. The writer forms each timestep's UVWs itself from the
+ * un-precessed tile positions (src/io/write/mod.rs:79-90): precessed to J2000 at precess_time's
+ * hadec_j2000, or at the LMST without precession. Host code: no device is needed. */
+typedef struct SmUvfitsParams {
+    int32_t num_timesteps;         /* groups in the file = num_timesteps x num_baselines */
+    int32_t num_baselines;         /* rows per timestep */
+    int32_t num_chans;
+    int32_t num_tiles;             /* rows of the AN table */
+    const char *const *tile_names; /* [num_tiles] */
+    const double *tile_xyzs;       /* [num_tiles][3] un-precessed geodetic XYZ, metres */
+    const int32_t *baseline_tiles; /* [num_baselines][2] tile indices, tile1 <= tile2 (equal: an auto) */
+    double first_freq_hz, freq_res_hz;
+    double phase_centre_ra, phase_centre_dec;
+    double array_longitude, array_latitude, array_height;
+    double dut1;                   /* seconds */
+    int32_t apply_precession;
+    double int_time_s;
+    double first_gps_seconds;      /* sets the file's whole-day date reference */
+    const char *obs_name;          /* nullable */
+} SmUvfitsParams;
+typedef struct SmUvfitsWriter SmUvfitsWriter;
+
+/* Opens the file and writes the primary header. */
+const char *sm_uvfits_create(const char *path, const SmUvfitsParams *params, SmUvfitsWriter **out);
+/* One timestep's groups: vis_fb host [num_chans][num_baselines] JonesF32, weights_fb host
+ * [num_chans][num_baselines] (nullable: all 1), at the timestep's centroid GPS time. */
+const char *sm_uvfits_write_timestep(SmUvfitsWriter *w, double gps_seconds, const JonesF32 *vis_fb,
+                                     const float *weights_fb);
+/* Writes the AIPS AN table, closes the file and frees the writer (also after an error). */
+const char *sm_uvfits_finish(SmUvfitsWriter *w);
 
 /* ------------------------------------------------------------------ peel-loop neighbours
  * The device work around the modeller in the reference's peel loop

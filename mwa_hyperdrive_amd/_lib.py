@@ -130,6 +130,30 @@ class SmPrecessionInfo(C.Structure):
     ]
 
 
+class SmUvfitsParams(C.Structure):
+    _fields_ = [
+        ("num_timesteps", C.c_int32),
+        ("num_baselines", C.c_int32),
+        ("num_chans", C.c_int32),
+        ("num_tiles", C.c_int32),
+        ("tile_names", C.c_void_p),
+        ("tile_xyzs", C.c_void_p),
+        ("baseline_tiles", C.c_void_p),
+        ("first_freq_hz", C.c_double),
+        ("freq_res_hz", C.c_double),
+        ("phase_centre_ra", C.c_double),
+        ("phase_centre_dec", C.c_double),
+        ("array_longitude", C.c_double),
+        ("array_latitude", C.c_double),
+        ("array_height", C.c_double),
+        ("dut1", C.c_double),
+        ("apply_precession", C.c_int32),
+        ("int_time_s", C.c_double),
+        ("first_gps_seconds", C.c_double),
+        ("obs_name", C.c_char_p),
+    ]
+
+
 class SmInfo(C.Structure):
     _fields_ = [
         ("num_baselines", C.c_int32),
@@ -198,6 +222,9 @@ SIGNATURES = {
     "sm_model_timestep_epoch_with": [_P, _D, _P, _P, _P],
     "sm_model_timestep_epoch_with_device": [_P, _D, _P, _P],
     "sm_model_timestep_autos_epoch_with": [_P, _D, _P, _P],
+    "sm_uvfits_create": [C.c_char_p, C.POINTER(SmUvfitsParams), C.POINTER(_P)],
+    "sm_uvfits_write_timestep": [_P, _D, _P, _P],
+    "sm_uvfits_finish": [_P],
 }
 # Entry points that return a value rather than an error string.
 VALUE_RETURN = {"sm_get_lmst": ([_D, _D, _D], C.c_double), "sm_surrogate_num_params": ([], C.c_int32)}

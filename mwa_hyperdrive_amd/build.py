@@ -20,7 +20,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libskymodel_hip.so"
 SOURCES = ["direct.hip", "direct_tiled.hip", "abi.hip", "modeller.hip", "factored.hip", "surrogate.hip", "peel.hip",
-           "precession.cpp"]
+           "precession.cpp", "uvfits.cpp"]
 ARCH = os.environ.get("SM_OFFLOAD_ARCH", "gfx950")
 
 

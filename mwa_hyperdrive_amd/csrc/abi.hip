@@ -356,6 +356,9 @@ hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf
         s.shape = in.shape;
         s.coh = coh.as<double>();
         s.bl_uv = in.bl_uv;
+        s.hadec = in.uv_hadec;
+        s.xyz = in.uv_xyz;
+        s.bl_tiles = in.uv_bl_tiles;
         s.prange = d_prange;
         s.packed = d_packed;
         s.sbf = ctx.d_sbf;

@@ -562,22 +562,15 @@ hipError_t launch_autos_demote(int n, const double *accum64, JonesF32 *vis32, Jo
 __global__ void k_shapelet_bl_uv(int ncomp, int ntiles, int nbl, const double *__restrict__ hadec,
                                  const double *__restrict__ xyz, const int2 *__restrict__ bl_tiles,
                                  double2 *__restrict__ out) {
-#pragma clang fp contract(off)
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (size_t)ncomp * nbl) return;
     const int c = (int)(idx / nbl);
     const int b = (int)(idx % nbl);
-    const double s_ha = hadec[4 * c], c_ha = hadec[4 * c + 1], s_dec = hadec[4 * c + 2];
+    const double4 hd = make_double4(hadec[4 * c], hadec[4 * c + 1], hadec[4 * c + 2], hadec[4 * c + 3]);
     const int2 t = bl_tiles[b];
-    const double x1 = xyz[3 * t.x], y1 = xyz[3 * t.x + 1], z1 = xyz[3 * t.x + 2];
-    const double x2 = xyz[3 * t.y], y2 = xyz[3 * t.y + 1], z2 = xyz[3 * t.y + 2];
-    const double c_dec = hadec[4 * c + 3];
-    const double u1 = s_ha * x1 + c_ha * y1;
-    const double v1 = -s_dec * c_ha * x1 + s_dec * s_ha * y1 + c_dec * z1;
-    const double u2 = s_ha * x2 + c_ha * y2;
-    const double v2 = -s_dec * c_ha * x2 + s_dec * s_ha * y2 + c_dec * z2;
     (void)ntiles;
-    out[idx] = make_double2(u1 - u2, v1 - v2);
+    out[idx] = shapelet_uv(hd, xyz[3 * t.x], xyz[3 * t.x + 1], xyz[3 * t.x + 2], xyz[3 * t.y], xyz[3 * t.y + 1],
+                           xyz[3 * t.y + 2]);
 }
 
 hipError_t launch_shapelet_bl_uv(int ncomp, int ntiles, int nbl, const double *d_comp_hadec, const double *d_xyz,

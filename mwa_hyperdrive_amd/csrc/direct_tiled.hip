@@ -767,6 +767,14 @@ hipError_t launch_direct_rect(Kind kind, const RectArgs &a, hipStream_t stream) 
 }
 
 // ------------------------------------------------------------------ shapelets
+// Lane `lane`'s value of v (every lane active), as a wave-uniform value.
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), lane);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 template <bool PER_TILE, class T>
 __device__ __forceinline__ void shapelet_chan(const ShapeletChanArgs &a) {
     const int nbx = (a.nbl + 3) / 4;
@@ -816,8 +824,39 @@ __device__ __forceinline__ void shapelet_chan(const ShapeletChanArgs &a) {
     // The wave's baseline is uniform; the next component's parameters are loaded one
     // iteration ahead.
     const int blu = __builtin_amdgcn_readfirstlane(bl);
+    // Shapelet UVs: formed here from the baseline's tile XYZs and each component's (HA, Dec)
+    // when those are given, so the [component][baseline] table is not re-read by every channel
+    // group; else read from bl_uv. The per-tile kernel reads bl_uv: it has no registers to spare
+    // (measured on the 256-tile case: more spills, 104 -> 110 ms per timestep).
+    const bool uv_here = !PER_TILE && a.hadec != nullptr;
+    double x1 = 0, y1 = 0, z1 = 0, x2 = 0, y2 = 0, z2 = 0;
+    if (uv_here) {
+        const int2 t = a.bl_tiles[blu];
+        x1 = a.xyz[3 * t.x];
+        y1 = a.xyz[3 * t.x + 1];
+        z1 = a.xyz[3 * t.x + 2];
+        x2 = a.xyz[3 * t.y];
+        y2 = a.xyz[3 * t.y + 1];
+        z2 = a.xyz[3 * t.y + 2];
+    }
+    const double4 *__restrict__ hd4 = reinterpret_cast<const double4 *>(a.hadec);
+    // The UV is wave-uniform: lane l forms the UV of component 64 q + l once per batch q of 64
+    // components, and each component reads its UV from that lane.
+    double2 uv_lane = make_double2(0.0, 0.0);
+    int uv_batch = -1;
+    auto uv_of = [&](int c) {
+        if (!uv_here) return a.bl_uv[(size_t)c * a.nbl + blu];
+        const int q = c >> 6;
+        if (q != uv_batch) {
+            const int cl = min((q << 6) + lane, a.ncomp - 1);
+            uv_lane = shapelet_uv(hd4[cl], x1, y1, z1, x2, y2, z2);
+            uv_batch = q;
+        }
+        const int src = c & 63;
+        return make_double2(readlane_f64(uv_lane.x, src), readlane_f64(uv_lane.y, src));
+    };
     double4 l_next = a.lmn[0], sh_next = a.shape[0];
-    double2 suv_next = a.bl_uv[blu];
+    double2 suv_next = uv_of(0);
     int4 pr_next = a.prange[0];
     for (int c = 0; c < a.ncomp; ++c) {
         const double4 l = l_next, sh = sh_next;
@@ -827,7 +866,7 @@ __device__ __forceinline__ void shapelet_chan(const ShapeletChanArgs &a) {
             const int cn = c + 1 < a.ncomp ? c + 1 : c;
             l_next = a.lmn[cn];
             sh_next = a.shape[cn];
-            suv_next = a.bl_uv[(size_t)cn * a.nbl + blu];
+            suv_next = uv_of(cn);
             pr_next = a.prange[cn];
         }
         const double theta = uvw.u * l.x + uvw.v * l.y + uvw.w * l.z;
@@ -991,13 +1030,16 @@ __device__ __forceinline__ void shapelet_chan(const ShapeletChanArgs &a) {
     }
 }
 
-__global__ __launch_bounds__(256) void k_shapelet_chan(const ShapeletChanArgs a) { shapelet_chan<false, double>(a); }
+// Shared beam: held to 128 VGPRs (4 waves per SIMD, no spills) instead of the 131 the compiler
+// picks with the UVs formed in the kernel (3 waves).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_shapelet_chan(const ShapeletChanArgs a) {
+    shapelet_chan<false, double>(a);
+}
 __global__ __launch_bounds__(256) void k_shapelet_chan32(const ShapeletChanArgs a) { shapelet_chan<false, float>(a); }
 
 // Per-tile beams: held to 128 VGPRs (4 waves per SIMD, a few spilled values) instead of the
 // 164 the compiler picks (3 waves). Measured on the 256-tile, 384-channel case: 129 -> 111 ms
-// per timestep. The shared-beam kernel keeps the compiler's choice: with the same cap it ran
-// 3.7% slower.
+// per timestep.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_shapelet_chan_tiles(
     const ShapeletChanArgs a) {
     shapelet_chan<true, double>(a);

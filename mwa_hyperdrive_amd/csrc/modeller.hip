@@ -818,6 +818,11 @@ static const char *model_timestep_device(SmModeller *m, double lst, double latit
         if (ty == SM_SHAPELET)
             if (const char *e = prepare_shapelet_uvs(m, d, lst)) return e;
         TypeInputs in = type_inputs(d);
+        if (ty == SM_SHAPELET) {
+            in.uv_hadec = d.hadec.as<double>();
+            in.uv_xyz = m->d_xyz.as<double>();
+            in.uv_bl_tiles = m->d_bl_tiles.as<int2>();
+        }
         const bool have_uvw = tile_uvw != nullptr && m->nt >= 2;
         const bool fact_uniform = ty == SM_POINT && factored_dtype >= 0 && ctx.beam.uniform_row >= 0 && have_uvw;
         const bool fact_group = ty == SM_POINT && factored_dtype >= 0 && ctx.beam.uniform_row < 0 &&
@@ -885,6 +890,7 @@ static const char *model_timestep_device(SmModeller *m, double lst, double latit
                 pj.shape = in.shape;
                 pj.coh = d.coh.as<double>();
                 pj.beam = ctx.beam;
+                pj.fp32 = true;  // the precision path: FP32 with FP64 phases, as its Gaussians
                 pj.vis32 = d_vis_fb;
                 pj.vis64 = d_vis_fb_f64;
                 pj.chunks = ctx.chunks;

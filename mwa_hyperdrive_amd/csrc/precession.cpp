@@ -16,6 +16,7 @@
 #include <cstring>
 
 #include "skymodel.h"
+#include "sm_time.h"
 
 namespace {
 
@@ -195,6 +196,11 @@ void to_radec(const double p[3], double &ra, double &dec) {
 }
 
 }  // namespace
+
+namespace sm {
+int tai_minus_utc_at(double gps_seconds) { return tai_minus_utc(gps_seconds); }
+double utc_jd_of_gps(double gps_seconds) { return utc_days(gps_seconds) + kJ2000Jd; }
+}  // namespace sm
 
 extern "C" double sm_get_lmst(double array_longitude_rad, double gps_seconds, double dut1_seconds) {
     return lmst_at(array_longitude_rad, gps_seconds, dut1_seconds);

@@ -12,6 +12,12 @@ struct TypeInputs {
     const double4 *lmn = nullptr;
     const double4 *shape = nullptr;
     const double2 *bl_uv = nullptr;
+    // What bl_uv is formed from, when known (the modeller): per component sin/cos of HA and
+    // Dec [n][4], tile XYZs [ntiles][3] and the tiles of each baseline. The channel-per-lane
+    // shapelet kernel then forms each UV in registers instead of reading bl_uv.
+    const double *uv_hadec = nullptr;
+    const double *uv_xyz = nullptr;
+    const int2 *uv_bl_tiles = nullptr;
     const int2 *coeff_range = nullptr;
     const ShapeletCoeff *coeffs = nullptr;
     int coeff_count = 0;  // total coefficients behind coeffs

@@ -113,6 +113,20 @@ __device__ __forceinline__ void triangle_pair(int b, int n, int &i, int &j) {
     j = b - start(r) + r + 1;
 }
 
+// Shapelet (u, v) of baseline (tile 1 - tile 2) with the component as phase centre
+// (ShapeletComponentParams::get_shapelet_uvws, src/srclist/types/components/mod.rs:299-315):
+// each tile's (u, v) at the component's (HA, Dec), then their difference. hd = (sin HA,
+// cos HA, sin Dec, cos Dec). No contraction, so every caller rounds alike.
+__device__ __forceinline__ double2 shapelet_uv(double4 hd, double x1, double y1, double z1, double x2, double y2,
+                                               double z2) {
+#pragma clang fp contract(off)
+    const double u1 = hd.x * x1 + hd.y * y1;
+    const double v1 = -hd.z * hd.y * x1 + hd.z * hd.x * y1 + hd.w * z1;
+    const double u2 = hd.x * x2 + hd.y * y2;
+    const double v2 = -hd.z * hd.y * x2 + hd.z * hd.x * y2 + hd.w * z2;
+    return make_double2(u1 - u2, v1 - v2);
+}
+
 // A shapelet coefficient as the kernels read it: the value with the sign of
 // i^((n1 + n2) mod 4) folded in, and the basis-table rows n1, n2. Per component the terms
 // that land in the real part of the envelope come first, then those that land in the

@@ -36,6 +36,11 @@ struct ShapeletChanArgs {
     const double4 *shape;
     const double *coh;
     const double2 *bl_uv;     // [ncomp][nbl]
+    // With hadec set, the UVs are formed in the kernel from hadec [ncomp][4] (sin HA, cos HA,
+    // sin Dec, cos Dec), xyz [ntiles][3] and bl_tiles [nbl], as k_shapelet_bl_uv forms bl_uv.
+    const double *hadec = nullptr;
+    const double *xyz = nullptr;
+    const int2 *bl_tiles = nullptr;
     const int4 *prange;
     const PackedCoeff *packed;
     const double *sbf;

@@ -3,8 +3,8 @@ reference's vis-simulate uvfits check (src/cli/vis_simulate/tests.rs:225-316) th
 epoch and precession handling.
 
 Inputs, read as data from the reference tree:
-- tile positions of observation 1090008640 from test_files/1090008640/1090008640.metafits
-  (binary table columns Antenna, Pol, East, North, Height), turned into geodetic XYZ at the
+- tile names and positions of observation 1090008640 from test_files/1090008640/1090008640.metafits
+  (binary table columns Antenna, Pol, TileName, East, North, Height), positions turned into geodetic XYZ at the
   MWA latitude as XyzGeodetic's ENH conversion does: x = -n sin(lat) + h cos(lat), y = e,
   z = n cos(lat) + h sin(lat);
 - the scheduled start (GPSTIME) and the phase centre (RAPHASE, DECPHASE);
@@ -69,15 +69,15 @@ def tile_xyzs(data: bytes):
             o, n, t = cols[name]
             b = rec[o:o + n * sizes[t]]
             if t == "A":
-                return b.decode().strip()
+                return b.decode().rstrip("\x00 ").strip()
             return struct.unpack({"I": ">h", "E": ">f"}[t], b)[0]
 
         if get("Pol") == "X":
-            rows.append((get("Antenna"), get("East"), get("North"), get("Height")))
+            rows.append((get("Antenna"), get("East"), get("North"), get("Height"), get("TileName")))
     rows.sort()
     lat = math.radians(MWA_LAT_DEG)
     s, c = math.sin(lat), math.cos(lat)
-    return [[-n * s + h * c, e, n * c + h * s] for _, e, n, h in rows], h0
+    return [[-n * s + h * c, e, n * c + h * s] for _, e, n, h, _ in rows], [r[4] for r in rows], h0
 
 
 def expected_group_params(text: str):
@@ -91,7 +91,7 @@ def expected_group_params(text: str):
 
 def main():
     data = METAFITS.read_bytes()
-    xyzs, h0 = tile_xyzs(data)
+    xyzs, names, h0 = tile_xyzs(data)
     cross, first_jd = expected_group_params(TESTS_RS.read_text())
     start = float(h0["GPSTIME"])
     out = {
@@ -102,6 +102,7 @@ def main():
         "dut1_seconds": 0.0,
         "timestamps_gps": [start + 4.0, start + 12.0],
         "tile_xyzs": xyzs,
+        "tile_names": names,
         "first_cross": {"tiles": [0, 1], "timestep": 0, "uvw_seconds": cross[0][:3], "baseline": cross[0][3]},
         "last_cross": {"tiles": [len(xyzs) - 2, len(xyzs) - 1], "timestep": 1, "uvw_seconds": cross[1][:3],
                        "baseline": cross[1][3]},

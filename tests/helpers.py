@@ -39,3 +39,40 @@ def f32_ulps(a, b):
     sp = np.spacing(np.abs(b)).astype(np.float64)
     sp = np.maximum(sp, np.spacing(np.float32(1e-30)))
     return float((np.abs(a.astype(np.float64) - b.astype(np.float64)) / sp).max())
+
+
+def vis_simulate_case():
+    """The reference's vis-simulate check (src/cli/vis_simulate/tests.rs:103-316) as fixtures:
+    (SourceList, tile XYZs, channel frequencies, phase centre, longitude, latitude, GPS times,
+    expected first / last cross rows' XX). The list is converted with the reference reader's
+    units (src/srclist/types/components/mod.rs:99-115): degrees * pi / 180 for RA, Dec and
+    position angle, arcseconds * pi / 180 / 3600 for Gaussian axes. The channels are the
+    metafits centre frequency and 80 kHz below it (src/cli/vis_simulate/mod.rs:463-480)."""
+    import json
+    import math
+    from pathlib import Path
+
+    from mwa_hyperdrive_amd.srclist import (FluxDensity, FluxList, Gaussian, Point, Source, SourceComponent,
+                                            SourceList, radec_from_degrees)
+
+    here = Path(__file__).parent / "golden"
+    g = json.loads((here / "vis_simulate_goldens.json").read_text())
+    pg = json.loads((here / "precession_goldens.json").read_text())
+    sl = SourceList()
+    for c in g["components"]:
+        ra, dec = radec_from_degrees(c["ra_deg"], c["dec_deg"])
+        if c["type"] == "point":
+            ct = Point()
+        else:
+            ct = Gaussian(math.radians(c["maj_arcsec"]) / 3600.0, math.radians(c["min_arcsec"]) / 3600.0,
+                          math.radians(c["pa_deg"]))
+        sl.setdefault(c["source"], Source()).components.append(
+            SourceComponent(ra, dec, ct, FluxList([FluxDensity(*f) for f in c["list"]])))
+    middle = g["metafits_freqcent_mhz"] * 1e6
+    n = g["num_fine_channels"]
+    freqs = np.array([middle - n / 2.0 * g["freq_res_hz"] + g["freq_res_hz"] * i for i in range(n)])
+    pc = tuple(math.radians(x) for x in pg["phase_centre_deg"])
+    return {"srclist": sl, "xyzs": np.asarray(pg["tile_xyzs"]), "freqs": freqs, "phase_centre": pc,
+            "longitude": math.radians(pg["array_longitude_deg"]), "latitude": math.radians(pg["array_latitude_deg"]),
+            "gps": pg["timestamps_gps"], "first": g["first_cross_xx_re_im_weight"][:2],
+            "last": g["last_cross_xx_re_im_weight"][:2]}

@@ -82,8 +82,9 @@ def test_factored_per_tile_beams(n_tiles, dead):
     Every pair with a tile of the most common beam row takes the MFMA path: pairs inside that
     row with coherency J0 S J0^H, pairs with one other-row tile r with J_r S J0^H (or its
     conjugate transpose when the other-row tile comes second). Pairs of two other-row tiles
-    take the FP64 direct kernel and must meet the FP64 bar on their own. 90 tiles put other-row
-    tiles on both sides of the class-0 tiles and across a 64-tile block boundary."""
+    take the own-beam pair kernel in FP32 with FP64 phases; they must meet the path's bar on
+    their own, and differ from FP64 (it is FP32 that ran). 90 tiles put other-row tiles on
+    both sides of the class-0 tiles and across a 64-tile block boundary."""
     obs = synthetic.shrink(synthetic.config(3, scale=100 / 20000), tiles_n=n_tiles, chans=9, timesteps=1)
     obs.beam = "analytic"
     obs.delays = np.zeros((n_tiles, 16), np.uint32)
@@ -111,10 +112,11 @@ def test_factored_per_tile_beams(n_tiles, dead):
 
     err_one, err_both = part(one), part(both)
     print(f"per-tile beams, {n_tiles} tiles, dead {dead}: rel err {err:.3e} "
-          f"(one other-row tile {err_one:.3e}, FP64 pairs {err_both:.3e})")
+          f"(one other-row tile {err_one:.3e}, two other-row tiles {err_both:.3e})")
     assert err <= FP16_TOL
     assert 0.0 < err_one <= FP16_TOL
-    assert err_both <= 1e-12
+    if both.any():
+        assert 1e-12 < err_both <= FP16_TOL
 
 
 def test_factored_c3_full_timestep_vs_fp64_direct():

@@ -52,10 +52,14 @@ __device__ __forceinline__ void exp_of(double x, float &out) {
     out = __builtin_amdgcn_exp2f((float)(x * 1.4426950408889634));
 }
 
-// sin and cos of an FP64 phase in the kernel's arithmetic. FP32: x is reduced mod pi/2 in FP64
-// exactly as fast_sincos does, and the reduced |r| <= pi/4 goes through the Taylor series of
-// sin to r^9 and of cos to r^10 in FP32 (truncation below 2e-9, under the FP32 rounding).
-__device__ __forceinline__ void sincos_phase(double x, double &s, double &c) { fast_sincos(x, s, c); }
+// (sin, cos)(k pi / 32) for table_sincos; 16-byte aligned, so a lane reads its pair in one load.
+static __device__ const double kSinCos32[128] __attribute__((aligned(16))) = {SM_SINCOS32_TABLE};
+
+// sin and cos of an FP64 phase in the kernel's arithmetic (the shapelet kernel's per-channel
+// phases). FP64: table_sincos. FP32: x is reduced mod pi/2 in FP64 exactly as fast_sincos does,
+// and the reduced |r| <= pi/4 goes through the Taylor series of sin to r^9 and of cos to r^10
+// in FP32 (truncation below 2e-9, under the FP32 rounding).
+__device__ __forceinline__ void sincos_phase(double x, double &s, double &c) { table_sincos(x, kSinCos32, s, c); }
 __device__ __forceinline__ void sincos_phase(double x, float &s, float &c) {
     const double k = rint(x * 0.6366197723675814);
     double r = fma(-k, 1.5707963267948966, x);
@@ -821,7 +825,7 @@ __device__ __forceinline__ void shapelet_chan(const ShapeletChanArgs &a) {
         fmax_w = fmax(fmax_w, __shfl_xor(fmax_w, o));
     }
     T acc[1][8] = {{0, 0, 0, 0, 0, 0, 0, 0}}, tot[1][8] = {{0, 0, 0, 0, 0, 0, 0, 0}};
-    // The wave's baseline is uniform; the next component's parameters are loaded one
+    // The wave's baseline is uniform; the next component's coefficient range is loaded one
     // iteration ahead.
     const int blu = __builtin_amdgcn_readfirstlane(bl);
     // Shapelet UVs: formed here from the baseline's tile XYZs and each component's (HA, Dec)
@@ -840,40 +844,46 @@ __device__ __forceinline__ void shapelet_chan(const ShapeletChanArgs &a) {
         z2 = a.xyz[3 * t.y + 2];
     }
     const double4 *__restrict__ hd4 = reinterpret_cast<const double4 *>(a.hadec);
-    // The UV is wave-uniform: lane l forms the UV of component 64 q + l once per batch q of 64
-    // components, and each component reads its UV from that lane.
-    double2 uv_lane = make_double2(0.0, 0.0);
-    int uv_batch = -1;
-    auto uv_of = [&](int c) {
-        if (!uv_here) return a.bl_uv[(size_t)c * a.nbl + blu];
-        const int q = c >> 6;
-        if (q != uv_batch) {
-            const int cl = min((q << 6) + lane, a.ncomp - 1);
-            uv_lane = shapelet_uv(hd4[cl], x1, y1, z1, x2, y2, z2);
-            uv_batch = q;
-        }
-        const int src = c & 63;
-        return make_double2(readlane_f64(uv_lane.x, src), readlane_f64(uv_lane.y, src));
-    };
-    double4 l_next = a.lmn[0], sh_next = a.shape[0];
-    double2 suv_next = uv_of(0);
-    int4 pr_next = a.prange[0];
-    for (int c = 0; c < a.ncomp; ++c) {
-        const double4 l = l_next, sh = sh_next;
-        const double2 suv = suv_next;
-        const int4 pr4 = pr_next;
-        {
-            const int cn = c + 1 < a.ncomp ? c + 1 : c;
-            l_next = a.lmn[cn];
-            sh_next = a.shape[cn];
-            suv_next = uv_of(cn);
-            pr_next = a.prange[cn];
-        }
-        const double theta = uvw.u * l.x + uvw.v * l.y + uvw.w * l.z;
-        T zr, zi;
-        sincos_phase(theta * fc, zi, zr);
+    // The phase dot product, the UV in the component's frame (xk, yk) and the table window are
+    // the same on every lane of the wave: lane l forms those of component 64 q + l once per
+    // batch q of 64 components, and each component reads them from that lane. The window is
+    // packed as first cell | width << 16 (both below SM_SBF_L < 2^15).
+    double th_lane = 0.0, xk_lane = 0.0, yk_lane = 0.0;
+    int wx_lane = 0, wy_lane = 0;
+    auto load_batch = [&](int q) {
+        const int cl = min((q << 6) + lane, a.ncomp - 1);
+        const double4 l = a.lmn[cl];
+        const double4 sh = a.shape[cl];
+        const double2 suv =
+            uv_here ? shapelet_uv(hd4[cl], x1, y1, z1, x2, y2, z2) : a.bl_uv[(size_t)cl * a.nbl + blu];
+        th_lane = uvw.u * l.x + uvw.v * l.y + uvw.w * l.z;
         const double xk = (suv.x * sh.x + suv.y * sh.y) * sh.z;
         const double yk = (suv.x * sh.y - suv.y * sh.x) * sh.w;
+        // The cells the wave touches: positions are monotone in frequency, so the extreme
+        // channels bound them (the same expressions as each lane's position, so the same
+        // roundings).
+        const double xa = xk * fmin_w + SM_SBF_C, xb = xk * fmax_w + SM_SBF_C;
+        const double ya = yk * fmin_w + SM_SBF_C, yb = yk * fmax_w + SM_SBF_C;
+        const int ix0 = (int)fmin(fmax(floor(fmin(xa, xb)), 0.0), (double)(SM_SBF_L - 2));
+        const int ix1 = (int)fmin(fmax(floor(fmax(xa, xb)), 0.0), (double)(SM_SBF_L - 2));
+        const int iy0 = (int)fmin(fmax(floor(fmin(ya, yb)), 0.0), (double)(SM_SBF_L - 2));
+        const int iy1 = (int)fmin(fmax(floor(fmax(ya, yb)), 0.0), (double)(SM_SBF_L - 2));
+        xk_lane = xk;
+        yk_lane = yk;
+        wx_lane = ix0 | ((ix1 - ix0 + 2) << 16);
+        wy_lane = iy0 | ((iy1 - iy0 + 2) << 16);
+    };
+    int4 pr_next = a.prange[0];
+    for (int c = 0; c < a.ncomp; ++c) {
+        const int4 pr4 = pr_next;
+        pr_next = a.prange[c + 1 < a.ncomp ? c + 1 : c];
+        if ((c & 63) == 0) load_batch(c >> 6);
+        const int src = c & 63;
+        const double theta = readlane_f64(th_lane, src);
+        const double xk = readlane_f64(xk_lane, src), yk = readlane_f64(yk_lane, src);
+        const int wxp = __builtin_amdgcn_readlane(wx_lane, src), wyp = __builtin_amdgcn_readlane(wy_lane, src);
+        T zr, zi;
+        sincos_phase(theta * fc, zi, zr);
         const double x_pos = xk * freq + SM_SBF_C;
         const double y_pos = yk * freq + SM_SBF_C;
         const double fx = floor(x_pos), fy = floor(y_pos);
@@ -881,22 +891,13 @@ __device__ __forceinline__ void shapelet_chan(const ShapeletChanArgs &a) {
         const int iy = (int)fmin(fmax(fy, 0.0), (double)(SM_SBF_L - 2));
         const T wx = (T)(x_pos - fx), wy = (T)(y_pos - fy);
         T er, ei;
-        // The cells the wave touches: positions are monotone in frequency, so the extreme
-        // channels bound them (the same expressions, so the same roundings).
-        const double xa = xk * fmin_w + SM_SBF_C, xb = xk * fmax_w + SM_SBF_C;
-        const double ya = yk * fmin_w + SM_SBF_C, yb = yk * fmax_w + SM_SBF_C;
-        const int ix0 = (int)fmin(fmax(floor(fmin(xa, xb)), 0.0), (double)(SM_SBF_L - 2));
-        const int ix1 = (int)fmin(fmax(floor(fmax(xa, xb)), 0.0), (double)(SM_SBF_L - 2));
-        const int iy0 = (int)fmin(fmax(floor(fmin(ya, yb)), 0.0), (double)(SM_SBF_L - 2));
-        const int iy1 = (int)fmin(fmax(floor(fmax(ya, yb)), 0.0), (double)(SM_SBF_L - 2));
-        const int xw = __builtin_amdgcn_readfirstlane(ix1 - ix0 + 2);
-        const int yw = __builtin_amdgcn_readfirstlane(iy1 - iy0 + 2);
+        const int xw = wxp >> 16, yw = wyp >> 16;
         T cpre[8];
         const T *s_c = coh_t + ((size_t)c * a.nchan_pad + f) * 8;
         if (xw <= kWin && yw <= kWin && pr4.w <= kRows) {
             // Stage rows 0 .. pr4.w - 1 of both windows in this wave's LDS, then interpolate
             // from LDS: one 16-byte read per axis per coefficient instead of two gathers.
-            const int bx0 = __builtin_amdgcn_readfirstlane(ix0), by0 = __builtin_amdgcn_readfirstlane(iy0);
+            const int bx0 = wxp & 0xffff, by0 = wyp & 0xffff;
             // All loads first, then all LDS writes: the loads are in flight together instead of
             // one round trip per row. A lane past the window or past row pr4.w - 1 reads an
             // in-bounds cell (the window's last cell; rows below kRows < SM_SBF_N) that is never

@@ -56,4 +56,70 @@ SM_HD inline void fast_sincos(double x, double &s, double &c) {
     c = ((q + 1) & 2) ? -b : b;
 }
 
+// (sin, cos) of k pi / 32 for k = 0 .. 63, each rounded to nearest from a 300-bit evaluation.
+#define SM_SINCOS32_TABLE \
+    0.0, 1.0, 0.0980171403295606, 0.9951847266721969, \
+    0.19509032201612828, 0.9807852804032304, 0.2902846772544624, 0.9569403357322088, \
+    0.3826834323650898, 0.9238795325112867, 0.47139673682599764, 0.881921264348355, \
+    0.5555702330196022, 0.8314696123025452, 0.6343932841636455, 0.773010453362737, \
+    0.7071067811865476, 0.7071067811865476, 0.773010453362737, 0.6343932841636455, \
+    0.8314696123025452, 0.5555702330196022, 0.881921264348355, 0.47139673682599764, \
+    0.9238795325112867, 0.3826834323650898, 0.9569403357322088, 0.2902846772544624, \
+    0.9807852804032304, 0.19509032201612828, 0.9951847266721969, 0.0980171403295606, \
+    1.0, 0.0, 0.9951847266721969, -0.0980171403295606, \
+    0.9807852804032304, -0.19509032201612828, 0.9569403357322088, -0.2902846772544624, \
+    0.9238795325112867, -0.3826834323650898, 0.881921264348355, -0.47139673682599764, \
+    0.8314696123025452, -0.5555702330196022, 0.773010453362737, -0.6343932841636455, \
+    0.7071067811865476, -0.7071067811865476, 0.6343932841636455, -0.773010453362737, \
+    0.5555702330196022, -0.8314696123025452, 0.47139673682599764, -0.881921264348355, \
+    0.3826834323650898, -0.9238795325112867, 0.2902846772544624, -0.9569403357322088, \
+    0.19509032201612828, -0.9807852804032304, 0.0980171403295606, -0.9951847266721969, \
+    0.0, -1.0, -0.0980171403295606, -0.9951847266721969, \
+    -0.19509032201612828, -0.9807852804032304, -0.2902846772544624, -0.9569403357322088, \
+    -0.3826834323650898, -0.9238795325112867, -0.47139673682599764, -0.881921264348355, \
+    -0.5555702330196022, -0.8314696123025452, -0.6343932841636455, -0.773010453362737, \
+    -0.7071067811865476, -0.7071067811865476, -0.773010453362737, -0.6343932841636455, \
+    -0.8314696123025452, -0.5555702330196022, -0.881921264348355, -0.47139673682599764, \
+    -0.9238795325112867, -0.3826834323650898, -0.9569403357322088, -0.2902846772544624, \
+    -0.9807852804032304, -0.19509032201612828, -0.9951847266721969, -0.0980171403295606, \
+    -1.0, 0.0, -0.9951847266721969, 0.0980171403295606, \
+    -0.9807852804032304, 0.19509032201612828, -0.9569403357322088, 0.2902846772544624, \
+    -0.9238795325112867, 0.3826834323650898, -0.881921264348355, 0.47139673682599764, \
+    -0.8314696123025452, 0.5555702330196022, -0.773010453362737, 0.6343932841636455, \
+    -0.7071067811865476, 0.7071067811865476, -0.6343932841636455, 0.773010453362737, \
+    -0.5555702330196022, 0.8314696123025452, -0.47139673682599764, 0.881921264348355, \
+    -0.3826834323650898, 0.9238795325112867, -0.2902846772544624, 0.9569403357322088, \
+    -0.19509032201612828, 0.9807852804032304, -0.0980171403295606, 0.9951847266721969
+
+// FP64 sin/cos with a table-driven reduction (P. T. P. Tang, "Table-lookup algorithms for
+// elementary functions and their error analysis", 10th IEEE Symposium on Computer Arithmetic,
+// 1991): x = k pi/32 + r with |r| <= pi/64 (1 + eps), then
+//   sin x = S_k cos r + C_k sin r,   cos x = C_k cos r - S_k sin r,
+// where (S_k, C_k) = (sin, cos)(k pi/32) come from `tab` (SM_SINCOS32_TABLE, index k mod 64)
+// and the Taylor series of sin r to r^9 and of cos r to r^8 leave remainders below 3e-20 at
+// |r| = pi/64. The reduction is fast_sincos's with a two-part pi/32 (P1 = pi/32 rounded to
+// double, P2 = the rest rounded to double, 32/pi rounded to double, all from a 300-bit pi).
+// Nine polynomial terms instead of twenty and no quadrant selection, for one table read.
+SM_HD inline void table_sincos(double x, const double *tab, double &s, double &c) {
+    const double k = rint(x * 10.185916357881302);
+    double r = fma(-k, 0.09817477042468103, x);
+    r = fma(-k, 3.827021247335479e-18, r);
+    const double r2 = r * r;
+    // coefficients (-1)^n / (2n+1)! and (-1)^n / (2n)!
+    double ps = 2.7557319223985893e-06;
+    ps = fma(ps, r2, -0.0001984126984126984);
+    ps = fma(ps, r2, 0.008333333333333333);
+    ps = fma(ps, r2, -0.16666666666666666);
+    const double sr = fma(r * r2, ps, r);
+    double pc = 2.48015873015873e-05;
+    pc = fma(pc, r2, -0.001388888888888889);
+    pc = fma(pc, r2, 0.041666666666666664);
+    pc = fma(pc, r2, -0.5);
+    const double cr = fma(r2, pc, 1.0);
+    const int i = (int)k & 63;  // k fits an int for |x| < 2e8, far beyond any phase here
+    const double S = tab[2 * i], C = tab[2 * i + 1];
+    s = fma(S, cr, C * sr);
+    c = fma(C, cr, -(S * sr));
+}
+
 }  // namespace sm

@@ -17,6 +17,10 @@ SRC = r"""
 extern "C" void run(int n, const double *x, double *s, double *c) {
     for (int i = 0; i < n; ++i) sm::fast_sincos(x[i], s[i], c[i]);
 }
+static const double kTab[128] = {SM_SINCOS32_TABLE};
+extern "C" void run_tab(int n, const double *x, double *s, double *c) {
+    for (int i = 0; i < n; ++i) sm::table_sincos(x[i], kTab, s[i], c[i]);
+}
 """
 
 
@@ -29,36 +33,44 @@ def lib(tmp_path_factory):
                     "-o", str(so), str(d / "t.cpp")], check=True)
     L = C.CDLL(str(so))
     L.run.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.run_tab.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
     return L
 
 
-def run(L, x):
+@pytest.fixture(scope="module", params=["fast_sincos", "table_sincos"])
+def fn(request, lib):
+    """Each test runs on both forms: fast_sincos (the tile kernels' phasors) and table_sincos
+    (the shapelet kernel's per-channel phases)."""
+    return lib.run if request.param == "fast_sincos" else lib.run_tab
+
+
+def run(f, x):
     x = np.ascontiguousarray(x, np.float64)
     s = np.empty_like(x)
     c = np.empty_like(x)
-    L.run(len(x), x.ctypes.data, s.ctypes.data, c.ctypes.data)
+    f(len(x), x.ctypes.data, s.ctypes.data, c.ctypes.data)
     return s, c
 
 
-def test_matches_libm_over_phase_range(lib):
+def test_matches_libm_over_phase_range(fn):
     rng = np.random.default_rng(0)
     x = np.concatenate([rng.uniform(-2e4, 2e4, 200000), rng.uniform(-4, 4, 50000),
                         rng.uniform(-1e6, 1e6, 20000)])
-    s, c = run(lib, x)
+    s, c = run(fn, x)
     assert np.abs(s - np.sin(x)).max() < 4e-16
     assert np.abs(c - np.cos(x)).max() < 4e-16
 
 
-def test_quadrant_boundaries_and_exact_points(lib):
+def test_quadrant_boundaries_and_exact_points(fn):
     k = np.arange(-4000, 4000, dtype=np.float64)
     for off in (0.0, 1e-9, -1e-9, np.pi / 4, -np.pi / 4):
         x = k * (np.pi / 2) + off
-        s, c = run(lib, x)
+        s, c = run(fn, x)
         assert np.abs(s - np.sin(x)).max() < 4e-16
         assert np.abs(c - np.cos(x)).max() < 4e-16
-    s, c = run(lib, np.array([0.0, -0.0]))
+    s, c = run(fn, np.array([0.0, -0.0]))
     assert s[0] == 0.0 and c[0] == 1.0 and c[1] == 1.0
     # Pythagorean identity far out.
     x = np.linspace(-3e4, 3e4, 10001)
-    s, c = run(lib, x)
+    s, c = run(fn, x)
     assert np.abs(s * s + c * c - 1.0).max() < 5e-16

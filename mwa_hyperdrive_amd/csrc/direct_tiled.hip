@@ -898,31 +898,38 @@ __device__ __forceinline__ void shapelet_chan(const ShapeletChanArgs &a) {
             // Stage rows 0 .. pr4.w - 1 of both windows in this wave's LDS, then interpolate
             // from LDS: one 16-byte read per axis per coefficient instead of two gathers.
             const int bx0 = wxp & 0xffff, by0 = wyp & 0xffff;
-            // All loads first, then all LDS writes: the loads are in flight together instead of
-            // one round trip per row. A lane past the window or past row pr4.w - 1 reads an
-            // in-bounds cell (the window's last cell; rows below kRows < SM_SBF_N) that is never
-            // written. Byte offsets are 32-bit (the table is 8 MB), from the uniform base.
-            const int j = lane & (kWin - 1), half = lane / kWin;
+            // Lane -> (row lane >> lw, cell lane & (2^lw - 1)), 2^lw the smallest power of two
+            // of at least 4 cells covering both windows, so one pass stages 64 >> lw rows of both
+            // axes (16 rows for windows of up to 4 cells). All loads first, then all LDS writes:
+            // the loads are in flight together instead of one round trip per row. A lane past
+            // the window or past row pr4.w - 1 reads an in-bounds cell (the window's last cell;
+            // rows below kRows + 16 <= SM_SBF_N) that is never written. Byte offsets are 32-bit
+            // (the table is 8 MB), from the uniform base.
+            const int wm = max(xw, yw);
+            const int lw = wm <= 4 ? 2 : (wm <= 8 ? 3 : (wm <= 16 ? 4 : 5));
+            const int rstep = 64 >> lw;
+            const int j = lane & ((1 << lw) - 1), rl = lane >> lw;
             const int jxc = min(j, xw - 1), jyc = min(j, yw - 1);
-            static_assert(kRows <= SM_SBF_N, "staged rows must exist in the basis table");
-            const unsigned ox = (unsigned)((half * SM_SBF_L + bx0 + jxc) * sizeof(T));
-            const unsigned oy = (unsigned)((half * SM_SBF_L + by0 + jyc) * sizeof(T));
+            static_assert(kRows + 16 <= SM_SBF_N, "staged rows must exist in the basis table");
+            const unsigned ox = (unsigned)((rl * SM_SBF_L + bx0 + jxc) * sizeof(T));
+            const unsigned oy = (unsigned)((rl * SM_SBF_L + by0 + jyc) * sizeof(T));
             const char *sbf_b = reinterpret_cast<const char *>(sbf);
             constexpr int kBatch = 2;  // row steps in flight at once (register budget)
-            for (int i0 = 0; i0 * (64 / kWin) < pr4.w; i0 += kBatch) {
+            for (int i0 = 0; i0 < pr4.w; i0 += kBatch * rstep) {
                 T tx[kBatch], ty[kBatch];
 #pragma unroll
                 for (int i = 0; i < kBatch; ++i) {
-                    const unsigned step = (unsigned)((i0 + i) * (64 / kWin) * SM_SBF_L * sizeof(T));
-                    if ((i0 + i) * (64 / kWin) < pr4.w) {
+                    const int base = i0 + i * rstep;
+                    const unsigned step = (unsigned)(base * SM_SBF_L * sizeof(T));
+                    if (base < pr4.w) {
                         tx[i] = *reinterpret_cast<const T *>(sbf_b + (ox + step));
                         ty[i] = *reinterpret_cast<const T *>(sbf_b + (oy + step));
                     }
                 }
 #pragma unroll
                 for (int i = 0; i < kBatch; ++i) {
-                    const int r = half + (i0 + i) * (64 / kWin);
-                    if ((i0 + i) * (64 / kWin) < pr4.w && r < pr4.w) {
+                    const int base = i0 + i * rstep, r = base + rl;
+                    if (base < pr4.w && r < pr4.w) {
                         if (j < xw) wx_lds[wv][r][j] = tx[i];
                         if (j < yw) wy_lds[wv][r][j] = ty[i];
                     }

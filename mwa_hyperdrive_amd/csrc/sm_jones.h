@@ -10,11 +10,12 @@ namespace sm {
 // The same for a wave-uniform p, through the constant address space so that the compiler
 // issues scalar loads (the coefficients are never written while a kernel runs).
 __device__ __forceinline__ void load_packed_uniform(const PackedCoeff *p, double &value, int &n1, int &n2) {
-    typedef __attribute__((address_space(4))) const int cint;
-    const cint *q = (const cint *)p;
-    value = __hiloint2double(q[1], q[0]);
-    n1 = q[2];
-    n2 = q[3];
+    typedef int int4v __attribute__((ext_vector_type(4)));
+    typedef __attribute__((address_space(4))) const int4v cint4v;
+    const int4v w = *(const cint4v *)p;
+    value = __hiloint2double(w.y, w.x);
+    n1 = w.z;
+    n2 = w.w;
 }
 
 __device__ __forceinline__ void load_packed(const PackedCoeff *p, double &value, int &n1, int &n2) {

@@ -614,32 +614,49 @@ hipError_t launch_repack_shapelet_uv(int nbl, int n_pl, int n_cpl, int n_list, c
 // ------------------------------------------------------------------ analytic beam
 // The product's MWA-like beam (see the header comment of sm_analytic_beam): 4x4 crossed
 // short dipoles, 1.1 m pitch, 0.278 m above a ground plane, 435 ps delay steps, delay 32 =
-// dead dipole. The dipole phase factorises into column x row terms, so a direction needs
-// 4 sincos for geometry; the 16 delay phasors of a (tile, freq) are built once per block.
-__global__ void k_analytic_beam(int n_freqs, const double *__restrict__ freqs, int n_dirs,
-                                const double *__restrict__ az, const double *__restrict__ za,
-                                const unsigned *__restrict__ delays, const double *__restrict__ gains,
-                                JonesF64 *__restrict__ out) {
-    __shared__ double s_wx[16][2], s_wy[16][2];  // gain * delay phasor per dipole (X, Y)
-    const int tf = blockIdx.y;                  // tile * n_freqs + freq
-    const int tile = tf / n_freqs;
-    const int fi = tf - tile * n_freqs;
-    const double freq = freqs[fi];
-    if (threadIdx.x < 16) {
-        const int d = threadIdx.x;
-        unsigned dl = delays[16 * tile + d];
-        double gx = gains[32 * tile + d], gy = gains[32 * tile + 16 + d];
-        if (dl == 32) {
-            gx = 0.0;
-            gy = 0.0;
-            dl = 0;
+// dead dipole. A block owns 256 directions of one tile and kBeamFB frequencies: each thread
+// forms its direction's geometry once and reuses it for every frequency. The dipole phase
+// factorises into column x row terms, exp(i k (x r_e + y r_n)) = cx(x) cy(y), with
+// x, y in {+-0.55, +-1.65} m; the 1.65 m terms follow from the 0.55 m ones by the triple-angle
+// identities, so a (direction, frequency) takes two sincos for geometry and one for the
+// ground-plane factor. The array factor sums row by row,
+//   AF = sum_row cy(row) sum_col w(row, col) cx(col),
+// with w = gain x delay phasor of each dipole, built per (tile, frequency) in LDS.
+constexpr int kBeamFB = 8;
+
+__global__ __launch_bounds__(256) void k_analytic_beam(int n_freqs, const double *__restrict__ freqs, int n_dirs,
+                                                       const double *__restrict__ az, const double *__restrict__ za,
+                                                       const unsigned *__restrict__ delays,
+                                                       const double *__restrict__ gains, JonesF64 *__restrict__ out) {
+    __shared__ double s_w[kBeamFB][16][4];  // per dipole: X re, X im, Y re, Y im
+    __shared__ double s_k[kBeamFB], s_norm[kBeamFB];
+    const int nfb = (n_freqs + kBeamFB - 1) / kBeamFB;
+    const int tile = blockIdx.y / nfb;
+    const int f0 = (blockIdx.y - tile * nfb) * kBeamFB;
+    const int nf = min(kBeamFB, n_freqs - f0);
+    if (threadIdx.x < 16 * kBeamFB) {
+        const int j = threadIdx.x >> 4, d = threadIdx.x & 15;
+        if (j < nf) {
+            const double freq = freqs[f0 + j];
+            unsigned dl = delays[16 * tile + d];
+            double gx = gains[32 * tile + d], gy = gains[32 * tile + 16 + d];
+            if (dl == 32) {
+                gx = 0.0;
+                gy = 0.0;
+                dl = 0;
+            }
+            double s, c;
+            sincos(-6.283185307179586 * freq * ((double)dl * 435e-12), &s, &c);
+            s_w[j][d][0] = gx * c;
+            s_w[j][d][1] = gx * s;
+            s_w[j][d][2] = gy * c;
+            s_w[j][d][3] = gy * s;
+            if (d == 0) {
+                const double k = 6.283185307179586 * freq / SM_VEL_C;
+                s_k[j] = k;
+                s_norm[j] = 1.0 / (16.0 * sin(k * 0.278));
+            }
         }
-        double s, c;
-        sincos(-6.283185307179586 * freq * ((double)dl * 435e-12), &s, &c);
-        s_wx[d][0] = gx * c;
-        s_wx[d][1] = gx * s;
-        s_wy[d][0] = gy * c;
-        s_wy[d][1] = gy * s;
     }
     __syncthreads();
     const int dir = blockIdx.x * blockDim.x + threadIdx.x;
@@ -648,58 +665,73 @@ __global__ void k_analytic_beam(int n_freqs, const double *__restrict__ freqs, i
     sincos(za[dir], &s_za, &c_za);
     sincos(az[dir], &s_az, &c_az);
     const double r_e = s_za * s_az, r_n = s_za * c_az, r_u = c_za;
-    const double k = 6.283185307179586 * freq / SM_VEL_C;
-    // Column phases exp(i k x r_e) for x = -1.65, -0.55, 0.55, 1.65 m; row phases for y.
-    double cx[4][2], cy[4][2];
-    {
-        double s1, c1, s3, c3;
-        sincos(k * 0.55 * r_e, &s1, &c1);
-        sincos(k * 1.65 * r_e, &s3, &c3);
-        cx[0][0] = c3; cx[0][1] = -s3;
-        cx[1][0] = c1; cx[1][1] = -s1;
-        cx[2][0] = c1; cx[2][1] = s1;
-        cx[3][0] = c3; cx[3][1] = s3;
-        sincos(k * 0.55 * r_n, &s1, &c1);
-        sincos(k * 1.65 * r_n, &s3, &c3);
-        // rows 0..3 have y = +1.65, +0.55, -0.55, -1.65 m
-        cy[0][0] = c3; cy[0][1] = s3;
-        cy[1][0] = c1; cy[1][1] = s1;
-        cy[2][0] = c1; cy[2][1] = -s1;
-        cy[3][0] = c3; cy[3][1] = -s3;
-    }
-    double ax_r = 0, ax_i = 0, ay_r = 0, ay_i = 0;
-#pragma unroll
-    for (int d = 0; d < 16; ++d) {
-        const int col = d & 3, row = d >> 2;
-        const double pr = cx[col][0] * cy[row][0] - cx[col][1] * cy[row][1];
-        const double pi = cx[col][0] * cy[row][1] + cx[col][1] * cy[row][0];
-        ax_r += s_wx[d][0] * pr - s_wx[d][1] * pi;
-        ax_i += s_wx[d][0] * pi + s_wx[d][1] * pr;
-        ay_r += s_wy[d][0] * pr - s_wy[d][1] * pi;
-        ay_i += s_wy[d][0] * pi + s_wy[d][1] * pr;
-    }
-    const double g = sin(k * 0.278 * r_u) / (16.0 * sin(k * 0.278));
     double sx = sqrt(1.0 - r_e * r_e);
     if (sx < 1e-12) sx = 1e-12;
-    const double px = g * sx, pyx = g * (-r_e * r_n / sx), pyy = g * (r_u / sx);
-    JonesF64 j;
-    j.j00_re = ax_r * px;
-    j.j00_im = ax_i * px;
-    j.j01_re = 0.0;
-    j.j01_im = 0.0;
-    j.j10_re = ay_r * pyx;
-    j.j10_im = ay_i * pyx;
-    j.j11_re = ay_r * pyy;
-    j.j11_im = ay_i * pyy;
-    out[(size_t)tf * n_dirs + dir] = j;
+    const double qyx = -r_e * r_n / sx, qyy = r_u / sx;
+    for (int j = 0; j < nf; ++j) {
+        const double k = s_k[j];
+        // Column phases for x = -1.65, -0.55, 0.55, 1.65 m; row phases for y = 1.65, 0.55,
+        // -0.55, -1.65 m (rows 0..3). sin 3a = s (3 - 4 s^2), cos 3a = c (4 c^2 - 3).
+        double cx[4][2], cy[4][2];
+        {
+            double s1, c1;
+            table_sincos(k * 0.55 * r_e, kSinCos32, s1, c1);
+            const double s3 = s1 * (3.0 - 4.0 * s1 * s1), c3 = c1 * (4.0 * c1 * c1 - 3.0);
+            cx[0][0] = c3; cx[0][1] = -s3;
+            cx[1][0] = c1; cx[1][1] = -s1;
+            cx[2][0] = c1; cx[2][1] = s1;
+            cx[3][0] = c3; cx[3][1] = s3;
+        }
+        {
+            double s1, c1;
+            table_sincos(k * 0.55 * r_n, kSinCos32, s1, c1);
+            const double s3 = s1 * (3.0 - 4.0 * s1 * s1), c3 = c1 * (4.0 * c1 * c1 - 3.0);
+            cy[0][0] = c3; cy[0][1] = s3;
+            cy[1][0] = c1; cy[1][1] = s1;
+            cy[2][0] = c1; cy[2][1] = -s1;
+            cy[3][0] = c3; cy[3][1] = -s3;
+        }
+        double ax_r = 0, ax_i = 0, ay_r = 0, ay_i = 0;
+#pragma unroll
+        for (int row = 0; row < 4; ++row) {
+            double rx_r = 0, rx_i = 0, ry_r = 0, ry_i = 0;
+#pragma unroll
+            for (int col = 0; col < 4; ++col) {
+                const double *w = s_w[j][4 * row + col];
+                rx_r += w[0] * cx[col][0] - w[1] * cx[col][1];
+                rx_i += w[0] * cx[col][1] + w[1] * cx[col][0];
+                ry_r += w[2] * cx[col][0] - w[3] * cx[col][1];
+                ry_i += w[2] * cx[col][1] + w[3] * cx[col][0];
+            }
+            ax_r += rx_r * cy[row][0] - rx_i * cy[row][1];
+            ax_i += rx_r * cy[row][1] + rx_i * cy[row][0];
+            ay_r += ry_r * cy[row][0] - ry_i * cy[row][1];
+            ay_i += ry_r * cy[row][1] + ry_i * cy[row][0];
+        }
+        double sg, cg;
+        table_sincos(k * 0.278 * r_u, kSinCos32, sg, cg);
+        const double g = sg * s_norm[j];
+        const double px = g * sx, pyx = g * qyx, pyy = g * qyy;
+        JonesF64 jo;
+        jo.j00_re = ax_r * px;
+        jo.j00_im = ax_i * px;
+        jo.j01_re = 0.0;
+        jo.j01_im = 0.0;
+        jo.j10_re = ay_r * pyx;
+        jo.j10_im = ay_i * pyx;
+        jo.j11_re = ay_r * pyy;
+        jo.j11_im = ay_i * pyy;
+        out[((size_t)tile * n_freqs + f0 + j) * n_dirs + dir] = jo;
+    }
 }
 
 hipError_t launch_analytic_beam(int n_tiles, const unsigned *d_delays, const double *d_gains, int n_freqs,
                                 const double *d_freqs, int n_dirs, const double *d_az, const double *d_za,
                                 JonesF64 *d_out, hipStream_t stream) {
     if (n_tiles <= 0 || n_freqs <= 0 || n_dirs <= 0) return hipSuccess;
-    if ((long)n_tiles * n_freqs > 65535) return hipErrorInvalidValue;
-    dim3 grid((n_dirs + 255) / 256, n_tiles * n_freqs);
+    const long ny = (long)n_tiles * ((n_freqs + kBeamFB - 1) / kBeamFB);
+    if (ny > 65535) return hipErrorInvalidValue;
+    dim3 grid((n_dirs + 255) / 256, (unsigned)ny);
     hipLaunchKernelGGL(k_analytic_beam, grid, dim3(256), 0, stream, n_freqs, d_freqs, n_dirs, d_az, d_za, d_delays,
                        d_gains, d_out);
     return hipGetLastError();

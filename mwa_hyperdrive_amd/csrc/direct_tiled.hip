@@ -52,9 +52,6 @@ __device__ __forceinline__ void exp_of(double x, float &out) {
     out = __builtin_amdgcn_exp2f((float)(x * 1.4426950408889634));
 }
 
-// (sin, cos)(k pi / 32) for table_sincos; 16-byte aligned, so a lane reads its pair in one load.
-static __device__ const double kSinCos32[128] __attribute__((aligned(16))) = {SM_SINCOS32_TABLE};
-
 // sin and cos of an FP64 phase in the kernel's arithmetic (the shapelet kernel's per-channel
 // phases). FP64: table_sincos. FP32: x is reduced mod pi/2 in FP64 exactly as fast_sincos does,
 // and the reduced |r| <= pi/4 goes through the Taylor series of sin to r^9 and of cos to r^10

@@ -122,4 +122,10 @@ SM_HD inline void table_sincos(double x, const double *tab, double &s, double &c
     c = fma(C, cr, -(S * sr));
 }
 
+#ifdef __HIPCC__
+// The device copy of SM_SINCOS32_TABLE for table_sincos, one per translation unit; 16-byte
+// aligned, so a lane reads its (sin, cos) pair in one load.
+[[maybe_unused]] static __device__ const double kSinCos32[128] __attribute__((aligned(16))) = {SM_SINCOS32_TABLE};
+#endif
+
 }  // namespace sm

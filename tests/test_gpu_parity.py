@@ -216,6 +216,32 @@ def test_analytic_beam_vs_oracle():
                 np.testing.assert_allclose(got[t, fi, d], want, rtol=0, atol=1e-13)
 
 
+def test_analytic_beam_vs_oracle_frequency_blocks():
+    """More frequencies than one beam block holds (8), in a count that leaves a partial block,
+    unevenly spaced, with a dead dipole: every (frequency, direction) against the oracle."""
+    import ctypes as C
+    from oracle import oracle as orc
+    from mwa_hyperdrive_amd import _lib
+    rng = np.random.default_rng(4)
+    n_dirs = 70
+    az = rng.uniform(0, 2 * np.pi, n_dirs)
+    za = rng.uniform(0, np.pi / 2, n_dirs)
+    freqs = np.sort(rng.uniform(70e6, 300e6, 19))
+    delays = rng.integers(0, 32, size=(1, 16)).astype(np.uint32)
+    delays[0, 9] = 32
+    gains = rng.uniform(0.5, 1.0, size=(1, 32))
+    d_az = torch.tensor(az, dtype=torch.float64, device="cuda")
+    d_za = torch.tensor(za, dtype=torch.float64, device="cuda")
+    out = torch.zeros((1, freqs.size, n_dirs, 8), dtype=torch.float64, device="cuda")
+    _lib.call("sm_analytic_beam", 1, delays.ctypes.data, gains.ctypes.data, freqs.size, freqs.ctypes.data, n_dirs,
+              C.c_void_p(d_az.data_ptr()), C.c_void_p(d_za.data_ptr()), C.c_void_p(out.data_ptr()))
+    got = out.cpu().numpy()
+    for fi, f in enumerate(freqs):
+        for d in range(n_dirs):
+            want = orc.analytic_beam(az[d], za[d], f, delays[0], gains[0])
+            np.testing.assert_allclose(got[0, fi, d], want, rtol=0, atol=1e-13)
+
+
 def test_device_buffer_entry_matches_host_entry():
     obs = _mixed_obs("analytic", n_tiles=16, comps=30)
     v32, v64, _, m = gpu_model(obs, 1)

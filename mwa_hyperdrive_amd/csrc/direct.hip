@@ -408,13 +408,16 @@ std::vector<int2> even_chunks(const double *freqs, int nchan, int width) {
     return out;
 }
 
-// Whether every chunk's channels are exactly f0 + k (f1 - f0); otherwise each channel gets
-// its own sincos/exp. Evaluated on the host copy of the padded frequencies.
+// Whether every fixed chunk's channels are exactly f0 + k (f1 - f0), for the chunk widths
+// 8 (and with it 4) and 6; otherwise each channel gets its own sincos/exp. Evaluated on the
+// host copy of the padded frequencies.
 bool chunks_even(const double *freqs, int nchan) {
-    for (int f0 = 0; f0 < nchan; f0 += SM_NCH) {
-        const double d = freqs[f0 + 1] - freqs[f0];
-        for (int k = 2; k < SM_NCH && f0 + k < nchan; ++k)
-            if (freqs[f0 + k] != freqs[f0] + (double)k * d) return false;
+    for (const int w : {SM_NCH, 6}) {
+        for (int f0 = 0; f0 < nchan; f0 += w) {
+            const double d = freqs[f0 + 1] - freqs[f0];
+            for (int k = 2; k < w && f0 + k < nchan; ++k)
+                if (freqs[f0 + k] != freqs[f0] + (double)k * d) return false;
+        }
     }
     return true;
 }

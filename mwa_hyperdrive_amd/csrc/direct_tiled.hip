@@ -25,12 +25,13 @@ constexpr int kTG = 16;   // tiles per group
 constexpr int kWin = 32;  // shapelet table window per wave and axis, cells
 constexpr int kRows = 16; // shapelet table rows staged per wave
 constexpr int kCT = 32;   // components per LDS stage
-// Channels per lane in k_direct_tiled. Fewer channels hold fewer accumulators (4 waves per
-// SIMD instead of 3 for points) but share each component's phasor products over fewer
+// Channels per lane in k_direct_tiled. Fewer channels hold fewer accumulators (more waves per
+// SIMD) but share each component's phasor products, and a Gaussian's three exps, over fewer
 // channels. Measured on C2 (768 channels): points 35.1 ms with 8, 33.6 ms with 4; Gaussians
-// 19.7 ms with 8, 21.7 ms with 4.
+// 19.4 ms with 8 (189 VGPRs, 2 waves per SIMD), 15.8 ms with 6 (157 VGPRs, 3 waves), 21.7 ms
+// with 4.
 constexpr int kTiledNchPoint = 4;
-constexpr int kTiledNchGaussian = 8;
+constexpr int kTiledNchGaussian = 6;
 static_assert(kTiledNchPoint <= SM_NCH && kTiledNchGaussian <= SM_NCH,
               "the padded channel count covers whole chunks of SM_NCH channels");
 

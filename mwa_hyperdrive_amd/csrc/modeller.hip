@@ -214,8 +214,8 @@ struct SmModeller {
     // device
     DevBuf d_freqs, d_xyz, d_bl_tiles, d_bl_rows, d_tile_rows, d_freq_map, d_ufreqs, d_delays, d_gains, d_sbf;
     DevBuf d_sbf32;  // the basis table in FP32 (precision path)
-    DevBuf d_chunks4, d_chunks8;  // evenly spaced channel runs of at most 4 and 8 channels
-    int nchunks4 = 0, nchunks8 = 0;
+    DevBuf d_chunks4, d_chunks6, d_chunks8;  // evenly spaced channel runs of at most 4, 6 and 8 channels
+    int nchunks4 = 0, nchunks6 = 0, nchunks8 = 0;
     DevBuf d_uvw, d_vis32, d_vis64, d_accum, d_tile_uvw, d_pairs;
     // split-precision point path
     DevBuf d_fblocks, d_fblock_of, d_fout, d_fscratch, d_fkbuf, d_fcohf, d_fmap;
@@ -561,10 +561,13 @@ extern "C" const char *sm_new(const SmObsParams *p, const SmComponent *comps, co
     SM_NEW_TRY(upload(m->d_sbf, shapelet_table()));
     {
         const std::vector<int2> c4 = even_chunks(m->freqs_pad.data(), m->nchan, 4);
+        const std::vector<int2> c6 = even_chunks(m->freqs_pad.data(), m->nchan, 6);
         const std::vector<int2> c8 = even_chunks(m->freqs_pad.data(), m->nchan, 8);
         m->nchunks4 = (int)c4.size();
+        m->nchunks6 = (int)c6.size();
         m->nchunks8 = (int)c8.size();
         SM_NEW_TRY(upload(m->d_chunks4, c4));
+        SM_NEW_TRY(upload(m->d_chunks6, c6));
         SM_NEW_TRY(upload(m->d_chunks8, c8));
     }
     SM_NEW_TRY(m->d_sbf32.reserve(sizeof(float) * (size_t)SM_SBF_N * SM_SBF_L));
@@ -776,6 +779,8 @@ static const char *model_timestep_device(SmModeller *m, double lst, double latit
     // whatever channels are flagged.
     ctx.chunks.c4 = m->d_chunks4.as<int2>();
     ctx.chunks.n4 = m->nchunks4;
+    ctx.chunks.c6 = m->d_chunks6.as<int2>();
+    ctx.chunks.n6 = m->nchunks6;
     ctx.chunks.c8 = m->d_chunks8.as<int2>();
     ctx.chunks.n8 = m->nchunks8;
     ctx.even = true;

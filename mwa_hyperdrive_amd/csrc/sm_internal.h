@@ -60,13 +60,16 @@ struct BeamDesc {
     int row0 = -1;
 };
 
-// Channel chunks of the direct kernels. With tables: runs of at most 4 (c4) or 8 (c8)
+// Channel chunks of the direct kernels. With tables: runs of at most 4 (c4), 6 (c6) or 8 (c8)
 // evenly spaced channels, (first channel, count), so that a flagged channel only shortens
-// the runs around it; every chunk is evenly spaced. Null tables: fixed chunks of 4 or 8
-// consecutive channels, evenly spaced or not.
+// the runs around it; every chunk is evenly spaced. Null tables: fixed chunks of 4, 6 or 8
+// consecutive channels, evenly spaced or not. A chunk reads up to 8 channels from its first
+// one, inside the padded channel count (pad_channels adds 8).
 struct ChunkTabs {
     const int2 *c4 = nullptr;
     int n4 = 0;
+    const int2 *c6 = nullptr;
+    int n6 = 0;
     const int2 *c8 = nullptr;
     int n8 = 0;
 };
@@ -75,18 +78,18 @@ struct ChunkTabs {
 // - freqs[f0]) exactly, covering all nchan channels in order.
 std::vector<int2> even_chunks(const double *freqs, int nchan, int width);
 
-// Number of chunks of width nch (4 or 8).
+// Number of chunks of width nch (4, 6 or 8).
 __host__ __device__ inline int chunk_count(const ChunkTabs &t, int nch, int nchan) {
-    const int2 *tab = nch == 4 ? t.c4 : t.c8;
-    if (tab != nullptr) return nch == 4 ? t.n4 : t.n8;
+    const int2 *tab = nch == 4 ? t.c4 : (nch == 6 ? t.c6 : t.c8);
+    if (tab != nullptr) return nch == 4 ? t.n4 : (nch == 6 ? t.n6 : t.n8);
     return (nchan + nch - 1) / nch;
 }
 
 // First channel and channel count of chunk `chunk` of width NCH.
 template <int NCH>
 __device__ __forceinline__ void chunk_range(const ChunkTabs &t, int chunk, int nchan, int &f0, int &cnt) {
-    static_assert(NCH == 4 || NCH == 8, "chunk tables exist for widths 4 and 8");
-    const int2 *tab = NCH == 4 ? t.c4 : t.c8;
+    static_assert(NCH == 4 || NCH == 6 || NCH == 8, "chunk tables exist for widths 4, 6 and 8");
+    const int2 *tab = NCH == 4 ? t.c4 : (NCH == 6 ? t.c6 : t.c8);
     if (tab != nullptr) {
         const int2 c = tab[chunk];
         f0 = c.x;

@@ -147,11 +147,23 @@ def addresses(freqs, nbl, n_ufreq, tile_map, freq_map, unflagged, sbf=None):
 @pytest.mark.parametrize("kind", ["point", "gaussian", "shapelet"])
 @pytest.mark.parametrize("beam", ["identity", "per_tile"])
 def test_dropin_vs_oracle(kind, beam, variant):
+    _dropin_case(kind, beam, variant, 167e6 + 40e3 * np.arange(19))
+
+
+@pytest.mark.parametrize("kind", ["point", "gaussian", "shapelet"])
+def test_dropin_gapped_channels_vs_oracle(kind, variant):
+    """Two runs of 8 evenly spaced channels with a gap of two channels between them, then 3
+    more: every fixed chunk of 8 (and 4) is evenly spaced, but a chunk of 6 channels straddles
+    the gap, so the drop-ins must give every channel its own phase."""
+    freqs = 167e6 + 40e3 * np.concatenate([np.arange(8), 10 + np.arange(8), 20 + np.arange(3)])
+    _dropin_case(kind, "identity", variant, freqs)
+
+
+def _dropin_case(kind, beam, variant, freqs):
     from oracle import oracle as orc
     rng = np.random.default_rng(11)
     nt = 12
     xyzs = synthetic.tiles(rng, nt, 800.0)
-    freqs = 167e6 + 40e3 * np.arange(19)
     pc = synthetic.PHASE_CENTRE
     lst = 0.3
     comps = make_sky(rng, 4, freqs, kind)

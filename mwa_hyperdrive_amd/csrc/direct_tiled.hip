@@ -275,14 +275,15 @@ std::vector<int4> tile_pairs(int ntiles) {
 hipError_t launch_direct_tiled(Kind kind, const TiledArgs &a_in, hipStream_t stream) {
     if (a_in.ncomp <= 0 || a_in.nbl <= 0 || a_in.nchan <= 0) return hipSuccess;
     TiledArgs a = a_in;
-    // FP32 Gaussians take 4 channels per lane: their exponentials are cheap hardware exp2
-    // calls, and the two-level accumulators of 8 channels would double the registers.
-    const int nch = kind == KIND_GAUSSIAN && a.coh32 == nullptr ? kTiledNchGaussian : kTiledNchPoint;
+    // FP32 Gaussians take 6 channels per lane too (142 VGPRs with the two-level accumulators,
+    // 3 waves per SIMD). Measured per C5 timestep: 78.4 ms with 4, 76.2 ms with 6, 115.9 ms
+    // with 8.
+    const int nch = kind == KIND_GAUSSIAN ? kTiledNchGaussian : kTiledNchPoint;
     a.nchunks = chunk_count(a.chunks, nch, a.nchan);
     const int total = a.npairs * a.nchunks;
     if (a.coh32 != nullptr) {
         if (kind == KIND_GAUSSIAN)
-            hipLaunchKernelGGL((k_direct_tiled<KIND_GAUSSIAN, kTiledNchPoint, float>), dim3(total), dim3(256), 0,
+            hipLaunchKernelGGL((k_direct_tiled<KIND_GAUSSIAN, kTiledNchGaussian, float>), dim3(total), dim3(256), 0,
                                stream, a);
         else
             hipLaunchKernelGGL((k_direct_tiled<KIND_POINT, kTiledNchPoint, float>), dim3(total), dim3(256), 0, stream,
@@ -589,9 +590,10 @@ hipError_t launch_pair_jones(Kind kind, const PairJonesArgs &a_in, DevBuf *accum
 // instrumental coherency S is Hermitian (XY and YX are conjugates).
 constexpr int kRC = 16;  // components per LDS stage
 constexpr int kRT = 4 + 64;  // tile slots per block: 4 rows, then 64 columns
-// Gaussians here take 4 channels per lane (2 -> 3 waves per SIMD), unlike the tile-pair
-// kernel: on the 256-tile, 384-channel case 236 -> 227 ms per timestep.
-constexpr int kRectNchGaussian = 4;
+// Gaussians here take 6 channels per lane. Measured per C5 timestep (256 tiles, 384 channels),
+// FP64: 48.1 ms with 4, 42.4 ms with 6 (151 VGPRs, 3 waves per SIMD); FP32: 28.3 ms with 4,
+// 26.8 ms with 6.
+constexpr int kRectNchGaussian = 6;
 
 template <int KIND, int NCH, class T>
 __global__ __launch_bounds__(256) void k_direct_rect(const RectArgs a) {
@@ -1050,7 +1052,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const ShapeletChanArgs a) {
     shapelet_chan<true, double>(a);
 }
-__global__ __launch_bounds__(256) void k_shapelet_chan_tiles32(const ShapeletChanArgs a) {
+// FP32 per-tile beams: held to 96 VGPRs (5 waves per SIMD, a few spilled values) instead of 118
+// (4 waves). Measured on C5: 66.2 -> 57.8 ms per timestep.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_shapelet_chan_tiles32(
+    const ShapeletChanArgs a) {
     shapelet_chan<true, float>(a);
 }
 

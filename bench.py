@@ -37,13 +37,18 @@ sys.path.insert(0, str(ROOT))
 METRIC = "Gcomp·vis/s, FP64 direct vs split-precision MFMA at ≤1e-6 rel err vs FP64"
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
-# Algorithmic FP64 work of the point tile-pair kernel per comp.vis: 4 complex multiply-adds of
-# coherency and phasor (32), the phasor's advance across the chunk's channels (6), the
-# tile-phasor product at the chunk's first channel shared by its 4 channels (1.5), and the
-# tile phasors themselves, shared by 256 baselines (under 0.5). The reference kernel's per
-# comp.vis J S J^H (112 FLOP) and per-baseline sincos are not part of this algorithm: J S J^H
-# is formed once per (component, channel) and the phase factorises over antennas.
-FLOP_PER_COMPVIS_POINT = 40.0
+# Algorithmic FP64 work of the point tile-pair kernel per comp.vis: the 4 Jones entries'
+# coherency x phasor accumulations, which with one beam on both sides take the Hermitian form
+# (XX and YY real, YX = conj(XY): Re XX, Re XY, Im XY and Re YY times the complex phasor, 8
+# FMAs = 16), the phasor's advance across the chunk's channels (6), the tile-phasor product at
+# the chunk's first channel shared by its 4 channels (1.5), and the tile phasors themselves,
+# shared by 256 baselines (under 0.5). The reference kernel's per comp.vis J S J^H (112 FLOP)
+# and per-baseline sincos are not part of this algorithm: J S J^H is formed once per
+# (component, channel) and the phase factorises over antennas.
+FLOP_PER_COMPVIS_POINT = 24.0
+# The same terms in the full complex form (4 complex multiply-adds = 32 instead of 16), as the
+# shapelet kernel computes them.
+FLOP_PER_COMPVIS_POINT_FULL = 40.0
 
 
 def cpu_model_name() -> str:
@@ -150,7 +155,7 @@ def shapelet_flop(obs) -> float:
     counts = [len(c.comp_type.coeffs) for src in obs.srclist.values() for c in src.components
               if isinstance(c.comp_type, Shapelet)]
     mean = sum(counts) / len(counts) if counts else 0.0
-    return FLOP_PER_COMPVIS_POINT + 34.0 + 9.0 * mean
+    return FLOP_PER_COMPVIS_POINT_FULL + 34.0 + 9.0 * mean
 
 
 def split_precision() -> dict:

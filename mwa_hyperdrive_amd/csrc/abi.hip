@@ -137,8 +137,13 @@ hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf
     hipError_t e = coh.reserve((size_t)n * ctx.nchan_pad * 8 * sizeof(double));
     if (e != hipSuccess) return e;
     const bool per_tile = ctx.beam.jones != nullptr && ctx.beam.uniform_row < 0;
+    // Shared beam, evenly spaced channels: the tile-pair kernel, which reads the table in its
+    // Hermitian form (4 values per entry).
+    const bool shared_tiled = ctx.d_bl_subset == nullptr && g_direct_variant == 0 && kind != KIND_SHAPELET &&
+                              !per_tile && ctx.even && ctx.d_tile_uvw != nullptr && ctx.d_pairs != nullptr &&
+                              ctx.ntiles >= 2;
     e = launch_coherency(n, ctx.nchan, ctx.nchan_pad, ctx.d_freqs_pad, in.fb, ctx.beam, !per_tile, coh.as<double>(),
-                         stream);
+                         stream, shared_tiled);
     if (e != hipSuccess) return e;
     // Per-tile shapelets with beam-row classes: J_r S J_0^H per class for the channel-per-lane
     // kernel, when the table fits (otherwise the kernel forms J_p S J_q^H per visibility).
@@ -197,13 +202,15 @@ hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf
     if (g0_tiled) {
         // Pairs of two class-0 tiles: J_0 S J_0^H, shared by all of them, on the tile-pair
         // kernel over the class-0 tiles; the other baselines: per-tile kernel on a subset.
+        // J_0 S J_0^H in the Hermitian form the tile-pair kernel reads (4 values per entry).
         DevBuf &coh0 = scratch.coh0;
-        e = coh0.reserve((size_t)n * ctx.nchan_pad * 8 * sizeof(double));
+        e = coh0.reserve((size_t)n * ctx.nchan_pad * 4 * sizeof(double));
         if (e != hipSuccess) return e;
         BeamDesc b0 = ctx.beam;
         b0.uniform_row = ctx.class_row0;
-        e = launch_coherency(n, ctx.nchan, ctx.nchan_pad, ctx.d_freqs_pad, in.fb, b0, true, coh0.as<double>(), stream);
-        if (e == hipSuccess && f32) e = to_f32(scratch.coh0_32, coh_n, coh0.as<double>(), stream);
+        e = launch_coherency(n, ctx.nchan, ctx.nchan_pad, ctx.d_freqs_pad, in.fb, b0, true, coh0.as<double>(), stream,
+                             true);
+        if (e == hipSuccess && f32) e = to_f32(scratch.coh0_32, coh_n / 2, coh0.as<double>(), stream);
         if (e != hipSuccess) return e;
         TiledArgs t;
         t.ntiles = ctx.g0_n;
@@ -318,8 +325,7 @@ hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf
     if (ev0 != nullptr) (void)hipEventRecord(ev0, stream);
     if (ctx.d_bl_subset != nullptr) {
         e = launch_subset(kind, per_tile, ctx.even, a, scratch.accum, stream);
-    } else if (g_direct_variant == 0 && kind != KIND_SHAPELET && !per_tile && ctx.even && ctx.d_tile_uvw != nullptr &&
-        ctx.d_pairs != nullptr && ctx.ntiles >= 2) {
+    } else if (shared_tiled) {
         TiledArgs t;
         t.ntiles = ctx.ntiles;
         t.nbl = ctx.nbl;
@@ -335,7 +341,7 @@ hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf
         t.shape = in.shape;
         t.coh = coh.as<double>();
         if (f32) {
-            e = to_f32(scratch.coh32, coh_n, coh.as<double>(), stream);
+            e = to_f32(scratch.coh32, coh_n / 2, coh.as<double>(), stream);
             if (e != hipSuccess) return e;
             t.coh32 = scratch.coh32.as<float>();
         }

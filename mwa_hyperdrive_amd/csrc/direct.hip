@@ -23,17 +23,18 @@ namespace sm {
 
 // ------------------------------------------------------------------ coherency table
 // coh[c][f] = J S J^H (one shared beam row) or S; S from the power-law / curved-power-law
-// parameters at 150 MHz or the list table (src/gpu/model.cu:16-28,137-167).
+// parameters at 150 MHz or the list table (src/gpu/model.cu:16-28,137-167). With `hermitian`,
+// 4 values per entry instead of 8: Re XX, Re XY, Im XY, Re YY (the tile-pair kernels' form;
+// J S J^H with the same J on both sides is Hermitian, as S is).
 __global__ void k_coherency(int ncomp, int nchan, int nchan_pad, const double *__restrict__ freqs, FluxBuckets fb,
-                            BeamDesc beam, int apply_beam, double *__restrict__ coh) {
+                            BeamDesc beam, int apply_beam, int hermitian, double *__restrict__ coh) {
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (size_t)ncomp * nchan_pad) return;
     const int c = (int)(idx / nchan_pad);
     const int f = (int)(idx % nchan_pad);
-    double *out = coh + idx * 8;
+    double *out = coh + idx * (hermitian ? 4 : 8);
     if (f >= nchan) {
-#pragma unroll
-        for (int e = 0; e < 8; ++e) out[e] = 0.0;
+        for (int e = 0; e < (hermitian ? 4 : 8); ++e) out[e] = 0.0;
         return;
     }
     const double freq = freqs[f];
@@ -57,15 +58,23 @@ __global__ void k_coherency(int ncomp, int nchan, int nchan_pad, const double *_
         const int i = c - fb.n_pl - fb.n_cpl;
         load_jones(&fb.list_fds[(size_t)f * fb.n_list + i], s);
     }
+    double o[8];
     if (apply_beam && beam.jones != nullptr) {
-        double j[8], o[8];
+        double j[8];
         load_jones(&beam.jones[((size_t)beam.uniform_row * beam.n_ufreq + beam.freq_map[f]) * beam.n_dir + beam.dir_offset + c], j);
         sandwich(j, s, j, o);
-#pragma unroll
-        for (int e = 0; e < 8; ++e) out[e] = o[e];
     } else {
 #pragma unroll
-        for (int e = 0; e < 8; ++e) out[e] = s[e];
+        for (int e = 0; e < 8; ++e) o[e] = s[e];
+    }
+    if (hermitian) {
+        out[0] = o[0];
+        out[1] = o[2];
+        out[2] = o[3];
+        out[3] = o[6];
+    } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) out[e] = o[e];
     }
 }
 
@@ -142,12 +151,12 @@ hipError_t launch_class_coherency(int nclass, int ncomp, int nchan, int nchan_pa
 }
 
 hipError_t launch_coherency(int ncomp, int nchan, int nchan_pad, const double *d_freqs, FluxBuckets fb,
-                            BeamDesc beam, bool apply_beam, double *d_coh, hipStream_t stream) {
+                            BeamDesc beam, bool apply_beam, double *d_coh, hipStream_t stream, bool hermitian) {
     if (ncomp <= 0) return hipSuccess;
     const size_t n = (size_t)ncomp * nchan_pad;
     const unsigned blocks = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(k_coherency, dim3(blocks), dim3(256), 0, stream, ncomp, nchan, nchan_pad, d_freqs, fb, beam,
-                       apply_beam ? 1 : 0, d_coh);
+                       apply_beam ? 1 : 0, hermitian ? 1 : 0, d_coh);
     return hipGetLastError();
 }
 

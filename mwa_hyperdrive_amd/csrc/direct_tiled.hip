@@ -193,6 +193,11 @@ __global__ __launch_bounds__(256) void k_direct_tiled(const TiledArgs a) {
             s_ph[cc][slot] = ph;
         }
         __syncthreads();
+        // Component cc + 1's tile phasors, coherencies and Gaussian shape are loaded while
+        // component cc's products run. The explicit wait comes first: a scalar load's result
+        // can only be waited for with every outstanding scalar and LDS load, so the next
+        // component's loads are issued after it and are not waited for until the next
+        // iteration.
         for (int cc = 0; cc < nc; ++cc) {
             const int c = c0 + cc;
             const V4 A = s_ph[cc][sp];
@@ -209,7 +214,11 @@ __global__ __launch_bounds__(256) void k_direct_tiled(const TiledArgs a) {
                 exp_of(qf * (DF * (2.0 * F0 + DF)), er);
                 exp_of(qf * (2.0 * DF * DF), eg);
             }
-            const T *cc8 = coh + ((size_t)c * a.nchan_pad + f0) * 8;
+            // The table is the Hermitian form (Re XX, Re XY, Im XY, Re YY) of J S J^H (see
+            // TiledArgs), 32 contiguous bytes per channel. Per visibility: V_XX = sum C_XX z,
+            // A = sum Re(C_XY) z, B = sum Im(C_XY) z, V_YY = sum C_YY z (8 FMAs per component
+            // instead of 16); V_XY = A + i B and V_YX = A - i B are formed once at the end.
+            const T *cc4 = coh + ((size_t)c * a.nchan_pad + f0) * 4;
 #pragma unroll
             for (int k = 0; k < NCH; ++k) {
                 T pr_ = zr, pi_ = zi;
@@ -217,14 +226,15 @@ __global__ __launch_bounds__(256) void k_direct_tiled(const TiledArgs a) {
                     pr_ *= env;
                     pi_ *= env;
                 }
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const T cr = cc8[k * 8 + 2 * e], ci = cc8[k * 8 + 2 * e + 1];
-                    acc[k][2 * e] = fma_t(cr, pr_, acc[k][2 * e]);
-                    acc[k][2 * e] = fma_t(-ci, pi_, acc[k][2 * e]);
-                    acc[k][2 * e + 1] = fma_t(cr, pi_, acc[k][2 * e + 1]);
-                    acc[k][2 * e + 1] = fma_t(ci, pr_, acc[k][2 * e + 1]);
-                }
+                const T cxx = cc4[k * 4], cxyr = cc4[k * 4 + 1], cxyi = cc4[k * 4 + 2], cyy = cc4[k * 4 + 3];
+                acc[k][0] = fma_t(cxx, pr_, acc[k][0]);
+                acc[k][1] = fma_t(cxx, pi_, acc[k][1]);
+                acc[k][2] = fma_t(cxyr, pr_, acc[k][2]);
+                acc[k][3] = fma_t(cxyr, pi_, acc[k][3]);
+                acc[k][4] = fma_t(cxyi, pr_, acc[k][4]);
+                acc[k][5] = fma_t(cxyi, pi_, acc[k][5]);
+                acc[k][6] = fma_t(cyy, pr_, acc[k][6]);
+                acc[k][7] = fma_t(cyy, pi_, acc[k][7]);
                 const T nr = zr * rr - zi * ri;
                 zi = zr * ri + zi * rr;
                 zr = nr;
@@ -249,13 +259,18 @@ __global__ __launch_bounds__(256) void k_direct_tiled(const TiledArgs a) {
         const int f = f0 + k;
         if (k >= cnt) break;
         const size_t o = (size_t)f * a.nbl + bl;
+        T s8[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) s8[e] = two_level_sum(acc[k][e], tot[k][e]);
+        // XX, XY = A + i B, YX = A - i B, YY.
+        const T v[8] = {s8[0], s8[1], s8[2] - s8[5], s8[3] + s8[4], s8[2] + s8[5], s8[3] - s8[4], s8[6], s8[7]};
         float *v32 = reinterpret_cast<float *>(&a.vis32[o]);
 #pragma unroll
-        for (int e = 0; e < 8; ++e) v32[e] += (float)two_level_sum(acc[k][e], tot[k][e]);
+        for (int e = 0; e < 8; ++e) v32[e] += (float)v[e];
         if (a.vis64 != nullptr) {
             double *v64 = reinterpret_cast<double *>(&a.vis64[o]);
 #pragma unroll
-            for (int e = 0; e < 8; ++e) v64[e] += (double)two_level_sum(acc[k][e], tot[k][e]);
+            for (int e = 0; e < 8; ++e) v64[e] += (double)v[e];
         }
     }
 }

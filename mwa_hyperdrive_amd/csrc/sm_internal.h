@@ -176,7 +176,7 @@ hipError_t launch_pair_coherency(int ncomp, int nchan, int nchan_pad, const doub
 // Component ranges for a baseline subset of nsub baselines (1 when they fill the GPU).
 int direct_subset_split(int nsub, int nchan, int ncomp);
 hipError_t launch_coherency(int ncomp, int nchan, int nchan_pad, const double *d_freqs, FluxBuckets fb,
-                            BeamDesc beam, bool apply_beam, double *d_coh, hipStream_t stream);
+                            BeamDesc beam, bool apply_beam, double *d_coh, hipStream_t stream, bool hermitian = false);
 hipError_t launch_direct(Kind kind, bool per_tile, bool even, const DirectArgs &a, hipStream_t stream);
 bool chunks_even(const double *freqs_padded, int nchan);
 hipError_t launch_autos(int ntiles, int nchan, int ncomp, const double *d_coh, int nchan_pad, BeamDesc beam,

@@ -15,9 +15,13 @@ struct TiledArgs {
     const double *tile_uvw;   // [ntiles][3] metres
     const double4 *lmn;
     const double4 *shape;
-    const double *coh;        // [ncomp][nchan_pad][8]
-    // FP32 arithmetic (phases and their range reduction still FP64): the coherency table in
-    // FP32, [ncomp][nchan_pad][8]. Null selects the FP64 kernels.
+    // The Hermitian form of the coherency table, [ncomp][nchan_pad][4] = (Re XX, Re XY, Im XY,
+    // Re YY) of J S J^H (launch_coherency with `hermitian`). Valid because the same J stands on
+    // both sides (one shared beam, or class 0's beam for pairs of two class-0 tiles) and S is
+    // Hermitian, so XX and YY are real and YX = conj(XY).
+    const double *coh;
+    // FP32 arithmetic (phases and their range reduction still FP64): the same table in FP32.
+    // Null selects the FP64 kernels.
     const float *coh32 = nullptr;
     JonesF32 *vis32;
     JonesF64 *vis64;

@@ -142,8 +142,12 @@ hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf
     const bool shared_tiled = ctx.d_bl_subset == nullptr && g_direct_variant == 0 && kind != KIND_SHAPELET &&
                               !per_tile && ctx.even && ctx.d_tile_uvw != nullptr && ctx.d_pairs != nullptr &&
                               ctx.ntiles >= 2;
+    // Shared-beam shapelets on the two-channels-per-lane kernel read the Hermitian form too.
+    const bool shared_shapelet =
+        ctx.d_bl_subset == nullptr && g_direct_variant != 1 && kind == KIND_SHAPELET && !per_tile;
+    const bool hermitian = shared_tiled || shared_shapelet;
     e = launch_coherency(n, ctx.nchan, ctx.nchan_pad, ctx.d_freqs_pad, in.fb, ctx.beam, !per_tile, coh.as<double>(),
-                         stream, shared_tiled);
+                         stream, hermitian);
     if (e != hipSuccess) return e;
     // Per-tile shapelets with beam-row classes: J_r S J_0^H per class for the channel-per-lane
     // kernel, when the table fits (otherwise the kernel forms J_p S J_q^H per visibility).
@@ -372,7 +376,7 @@ hipError_t run_type(Kind kind, const TypeInputs &in, const ModelCtx &ctx, DevBuf
         s.vis32 = vis32;
         s.vis64 = vis64;
         if (f32 && ctx.d_sbf32 != nullptr) {
-            e = to_f32(scratch.coh32, coh_n, coh.as<double>(), stream);
+            e = to_f32(scratch.coh32, hermitian ? coh_n / 2 : coh_n, coh.as<double>(), stream);
             if (e == hipSuccess && beam.coh_cls != nullptr)
                 e = to_f32(scratch.cls32, (size_t)ctx.nclass * coh_n, beam.coh_cls, stream);
             if (e != hipSuccess) return e;

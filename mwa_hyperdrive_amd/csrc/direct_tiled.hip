@@ -1053,12 +1053,269 @@ __device__ __forceinline__ void shapelet_chan(const ShapeletChanArgs &a) {
     }
 }
 
-// Shared beam: held to 128 VGPRs (4 waves per SIMD, no spills) instead of the 131 the compiler
-// picks with the UVs formed in the kernel (3 waves).
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_shapelet_chan(const ShapeletChanArgs a) {
-    shapelet_chan<false, double>(a);
+// Shared beam: one wave per baseline and 128 channels, two adjacent channels per lane
+// (channels 2 l and 2 l + 1 of the group on lane l). The table cells are read for both channels
+// at once: adjacent channels' positions differ by |k| df, below one cell (checked per component),
+// so both fall in cells i0 and i0 + 1 with i0 the lower of their two cells, and the lane reads
+// the three values T[i0], T[i0 + 1], T[i0 + 2] per axis and coefficient instead of two values
+// per channel (three quarters of the LDS bytes). A channel in cell i0 interpolates
+//   u = T[i0] + (T[i0+1] - T[i0]) w,
+// as the single-channel form does, and one in cell i0 + 1 takes
+//   u = (T[i0] + (T[i0+1] - T[i0])) + (T[i0+2] - T[i0+1]) w,
+// equal to T[i0+1] + (T[i0+2] - T[i0+1]) w up to the rounding of T[i0] + (T[i0+1] - T[i0]).
+// The coherency table is in the Hermitian form (Re XX, Re XY, Im XY, Re YY; J S J^H with one
+// beam on both sides), so each channel accumulates with 8 FMAs per component.
+template <class T>
+__device__ __forceinline__ void shapelet_pair(const ShapeletChanArgs &a) {
+    const int ngroups = (a.nchan + 127) / 128;
+    const int nbx = (a.nbl + 3) / 4;
+    const int total = nbx * ngroups;
+    int b = blockIdx.x;
+    if ((total & 7) == 0) b = (b & 7) * (total >> 3) + (b >> 3);
+    const int group = b / nbx;
+    const int bx = b - group * nbx;
+    const int lane = threadIdx.x & 63;
+    const int bl = bx * 4 + (int)(threadIdx.x >> 6);
+    if (bl >= a.nbl) return;  // whole wave; the kernel has no barriers
+    const int fa_raw = group * 128 + 2 * lane;
+    const bool valid_a = fa_raw < a.nchan, valid_b = fa_raw + 1 < a.nchan;
+    const int fa = valid_a ? fa_raw : a.nchan - 1;
+    const int fb = valid_b ? fa_raw + 1 : a.nchan - 1;
+    const double freq_a = a.freqs[fa], freq_b = a.freqs[fb];
+    const double fc_a = freq_a / SM_VEL_C, fc_b = freq_b / SM_VEL_C;
+    const UVW uvw = a.uvw[bl];
+    const PackedCoeff *__restrict__ packed = a.packed;
+    const T *__restrict__ sbf = pick<T>(a.sbf, a.sbf32);
+    const T *__restrict__ coh_t = pick<T>(a.coh, a.coh32);
+    // Per-wave windows of the basis table (kRows rows x kWin cells per axis).
+    __shared__ T wx_lds[4][kRows][kWin], wy_lds[4][kRows][kWin];
+    const int wv = threadIdx.x >> 6;
+    // The wave's lowest and highest channel frequency and its largest step between a lane's
+    // two channels (channels past the last repeat it).
+    double fmin_w = fmin(freq_a, freq_b), fmax_w = fmax(freq_a, freq_b), dfmax_w = fabs(freq_b - freq_a);
+    for (int o = 32; o > 0; o >>= 1) {
+        fmin_w = fmin(fmin_w, __shfl_xor(fmin_w, o));
+        fmax_w = fmax(fmax_w, __shfl_xor(fmax_w, o));
+        dfmax_w = fmax(dfmax_w, __shfl_xor(dfmax_w, o));
+    }
+    T acc[2][8], tot[2][8];
+#pragma unroll
+    for (int ch = 0; ch < 2; ++ch)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) acc[ch][e] = tot[ch][e] = T(0);
+    const int blu = __builtin_amdgcn_readfirstlane(bl);
+    // Shapelet UVs: formed here from the baseline's tile XYZs and each component's (HA, Dec)
+    // when those are given; else read from bl_uv.
+    const bool uv_here = a.hadec != nullptr;
+    double x1 = 0, y1 = 0, z1 = 0, x2 = 0, y2 = 0, z2 = 0;
+    if (uv_here) {
+        const int2 t = a.bl_tiles[blu];
+        x1 = a.xyz[3 * t.x];
+        y1 = a.xyz[3 * t.x + 1];
+        z1 = a.xyz[3 * t.x + 2];
+        x2 = a.xyz[3 * t.y];
+        y2 = a.xyz[3 * t.y + 1];
+        z2 = a.xyz[3 * t.y + 2];
+    }
+    const double4 *__restrict__ hd4 = reinterpret_cast<const double4 *>(a.hadec);
+    // As in shapelet_chan: the wave-uniform per-component values, formed by lane l for
+    // component 64 q + l and read back by lane.
+    double th_lane = 0.0, xk_lane = 0.0, yk_lane = 0.0;
+    int wx_lane = 0, wy_lane = 0;
+    auto load_batch = [&](int q) {
+        const int cl = min((q << 6) + lane, a.ncomp - 1);
+        const double4 l = a.lmn[cl];
+        const double4 sh = a.shape[cl];
+        const double2 suv =
+            uv_here ? shapelet_uv(hd4[cl], x1, y1, z1, x2, y2, z2) : a.bl_uv[(size_t)cl * a.nbl + blu];
+        th_lane = uvw.u * l.x + uvw.v * l.y + uvw.w * l.z;
+        const double xk = (suv.x * sh.x + suv.y * sh.y) * sh.z;
+        const double yk = (suv.x * sh.y - suv.y * sh.x) * sh.w;
+        const double xa = xk * fmin_w + SM_SBF_C, xb = xk * fmax_w + SM_SBF_C;
+        const double ya = yk * fmin_w + SM_SBF_C, yb = yk * fmax_w + SM_SBF_C;
+        const int ix0 = (int)fmin(fmax(floor(fmin(xa, xb)), 0.0), (double)(SM_SBF_L - 2));
+        const int ix1 = (int)fmin(fmax(floor(fmax(xa, xb)), 0.0), (double)(SM_SBF_L - 2));
+        const int iy0 = (int)fmin(fmax(floor(fmin(ya, yb)), 0.0), (double)(SM_SBF_L - 2));
+        const int iy1 = (int)fmin(fmax(floor(fmax(ya, yb)), 0.0), (double)(SM_SBF_L - 2));
+        xk_lane = xk;
+        yk_lane = yk;
+        wx_lane = ix0 | ((ix1 - ix0 + 2) << 16);
+        wy_lane = iy0 | ((iy1 - iy0 + 2) << 16);
+    };
+    int4 pr_next = a.prange[0];
+    for (int c = 0; c < a.ncomp; ++c) {
+        const int4 pr4 = pr_next;
+        pr_next = a.prange[c + 1 < a.ncomp ? c + 1 : c];
+        if ((c & 63) == 0) load_batch(c >> 6);
+        const int src = c & 63;
+        const double theta = readlane_f64(th_lane, src);
+        const double xk = readlane_f64(xk_lane, src), yk = readlane_f64(yk_lane, src);
+        const int wxp = __builtin_amdgcn_readlane(wx_lane, src), wyp = __builtin_amdgcn_readlane(wy_lane, src);
+        T zr[2], zi[2];
+        sincos_phase(theta * fc_a, zi[0], zr[0]);
+        sincos_phase(theta * fc_b, zi[1], zr[1]);
+        const double xpa = xk * freq_a + SM_SBF_C, xpb = xk * freq_b + SM_SBF_C;
+        const double ypa = yk * freq_a + SM_SBF_C, ypb = yk * freq_b + SM_SBF_C;
+        const double fxa = floor(xpa), fxb = floor(xpb), fya = floor(ypa), fyb = floor(ypb);
+        const int ixa = (int)fmin(fmax(fxa, 0.0), (double)(SM_SBF_L - 2));
+        const int ixb = (int)fmin(fmax(fxb, 0.0), (double)(SM_SBF_L - 2));
+        const int iya = (int)fmin(fmax(fya, 0.0), (double)(SM_SBF_L - 2));
+        const int iyb = (int)fmin(fmax(fyb, 0.0), (double)(SM_SBF_L - 2));
+        const T wxa = (T)(xpa - fxa), wxb = (T)(xpb - fxb), wya = (T)(ypa - fya), wyb = (T)(ypb - fyb);
+        const int xw = wxp >> 16, yw = wyp >> 16;
+        // One component's coherencies for both channels (Hermitian form, 4 values each).
+        T cq[2][4];
+        {
+            const T *s_a = coh_t + ((size_t)c * a.nchan_pad + fa) * 4;
+            const T *s_b = coh_t + ((size_t)c * a.nchan_pad + fb) * 4;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                cq[0][e] = s_a[e];
+                cq[1][e] = s_b[e];
+            }
+        }
+        T er[2], ei[2];
+        // Staged path: both windows, one extra cell for T[i0 + 2], fit the LDS copy, and the
+        // two channels of a lane are less than half a cell apart on both axes (so their cells
+        // differ by at most one, with a wide margin for rounding).
+        const bool near = fabs(xk) * dfmax_w < 0.5 && fabs(yk) * dfmax_w < 0.5;
+        if (near && xw + 1 <= kWin && yw + 1 <= kWin && pr4.w <= kRows) {
+            const int bx0 = wxp & 0xffff, by0 = wyp & 0xffff;
+            const int xs = xw + 1, ys = yw + 1;  // staged cells per axis
+            // As in shapelet_chan: lane -> (row lane >> lw, cell lane & (2^lw - 1)). A lane past
+            // the window or past row pr4.w - 1 reads an in-bounds cell that is never written; the
+            // extra cell is at most SM_SBF_L, in the row or at the next row's first cell, and it
+            // is only multiplied by a zero weight when outside the row.
+            const int wm = max(xs, ys);
+            const int lw = wm <= 4 ? 2 : (wm <= 8 ? 3 : (wm <= 16 ? 4 : 5));
+            const int rstep = 64 >> lw;
+            const int j = lane & ((1 << lw) - 1), rl = lane >> lw;
+            const int jxc = min(j, xs - 1), jyc = min(j, ys - 1);
+            static_assert(kRows + 16 < SM_SBF_N, "staged rows and the next row must exist in the basis table");
+            const unsigned ox = (unsigned)((rl * SM_SBF_L + bx0 + jxc) * sizeof(T));
+            const unsigned oy = (unsigned)((rl * SM_SBF_L + by0 + jyc) * sizeof(T));
+            const char *sbf_b = reinterpret_cast<const char *>(sbf);
+            constexpr int kBatch = 2;
+            for (int i0 = 0; i0 < pr4.w; i0 += kBatch * rstep) {
+                T tx[kBatch], ty[kBatch];
+#pragma unroll
+                for (int i = 0; i < kBatch; ++i) {
+                    const int base = i0 + i * rstep;
+                    const unsigned step = (unsigned)(base * SM_SBF_L * sizeof(T));
+                    if (base < pr4.w) {
+                        tx[i] = *reinterpret_cast<const T *>(sbf_b + (ox + step));
+                        ty[i] = *reinterpret_cast<const T *>(sbf_b + (oy + step));
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < kBatch; ++i) {
+                    const int base = i0 + i * rstep, r = base + rl;
+                    if (base < pr4.w && r < pr4.w) {
+                        if (j < xs) wx_lds[wv][r][j] = tx[i];
+                        if (j < ys) wy_lds[wv][r][j] = ty[i];
+                    }
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // Each channel's weights on (T[i0+1] - T[i0]) and (T[i0+2] - T[i0+1]).
+            const int ix0l = min(ixa, ixb), iy0l = min(iya, iyb);
+            const int jx = ix0l - bx0, jy = iy0l - by0;
+            const bool xa_lo = ixa == ix0l, xb_lo = ixb == ix0l, ya_lo = iya == iy0l, yb_lo = iyb == iy0l;
+            const T ax_a = xa_lo ? wxa : T(1), bx_a = xa_lo ? T(0) : wxa;
+            const T ax_b = xb_lo ? wxb : T(1), bx_b = xb_lo ? T(0) : wxb;
+            const T ay_a = ya_lo ? wya : T(1), by_a = ya_lo ? T(0) : wya;
+            const T ay_b = yb_lo ? wyb : T(1), by_b = yb_lo ? T(0) : wyb;
+#pragma unroll
+            for (int pt = 0; pt < 2; ++pt) {
+                const PackedCoeff *cp = packed + pr4.x + (pt ? pr4.y : 0);
+                const int cnt = pt ? pr4.z : pr4.y;
+                T acc_a = T(0), acc_b = T(0);
+#pragma unroll 2
+                for (int k = 0; k < cnt; ++k) {
+                    double val;
+                    int n1, n2;
+                    load_packed_uniform(cp + k, val, n1, n2);
+                    const T x0 = wx_lds[wv][n1][jx], x1v = wx_lds[wv][n1][jx + 1], x2v = wx_lds[wv][n1][jx + 2];
+                    const T y0 = wy_lds[wv][n2][jy], y1v = wy_lds[wv][n2][jy + 1], y2v = wy_lds[wv][n2][jy + 2];
+                    const T dx01 = x1v - x0, dx12 = x2v - x1v, dy01 = y1v - y0, dy12 = y2v - y1v;
+                    const T u_a = fma_t(dx12, bx_a, fma_t(dx01, ax_a, x0));
+                    const T u_b = fma_t(dx12, bx_b, fma_t(dx01, ax_b, x0));
+                    const T v_a = fma_t(dy12, by_a, fma_t(dy01, ay_a, y0));
+                    const T v_b = fma_t(dy12, by_b, fma_t(dy01, ay_b, y0));
+                    acc_a += ((T)val * u_a) * v_a;
+                    acc_b += ((T)val * u_b) * v_b;
+                }
+                (pt ? ei : er)[0] = acc_a;
+                (pt ? ei : er)[1] = acc_b;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        } else {
+            // Rare: one coefficient at a time from the table in memory, per channel.
+            const int ixs[2] = {ixa, ixb}, iys[2] = {iya, iyb};
+            const T wxs[2] = {wxa, wxb}, wys[2] = {wya, wyb};
+#pragma unroll
+            for (int ch = 0; ch < 2; ++ch) {
+#pragma unroll
+                for (int pt = 0; pt < 2; ++pt) {
+                    const PackedCoeff *cp = packed + pr4.x + (pt ? pr4.y : 0);
+                    const int cnt = pt ? pr4.z : pr4.y;
+                    T accp = T(0);
+#pragma unroll 1
+                    for (int k = 0; k < cnt; ++k) {
+                        double val;
+                        int n1, n2;
+                        load_packed_uniform(cp + k, val, n1, n2);
+                        const T *bxp = sbf + n1 * SM_SBF_L + ixs[ch], *byp = sbf + n2 * SM_SBF_L + iys[ch];
+                        const T u_value = bxp[0] + (bxp[1] - bxp[0]) * wxs[ch];
+                        const T v_value = byp[0] + (byp[1] - byp[0]) * wys[ch];
+                        accp += ((T)val * u_value) * v_value;
+                    }
+                    (pt ? ei : er)[ch] = accp;
+                }
+            }
+        }
+#pragma unroll
+        for (int ch = 0; ch < 2; ++ch) {
+            const T pr = zr[ch] * er[ch] - zi[ch] * ei[ch];
+            const T pi = zr[ch] * ei[ch] + zi[ch] * er[ch];
+            acc[ch][0] = fma_t(cq[ch][0], pr, acc[ch][0]);
+            acc[ch][1] = fma_t(cq[ch][0], pi, acc[ch][1]);
+            acc[ch][2] = fma_t(cq[ch][1], pr, acc[ch][2]);
+            acc[ch][3] = fma_t(cq[ch][1], pi, acc[ch][3]);
+            acc[ch][4] = fma_t(cq[ch][2], pr, acc[ch][4]);
+            acc[ch][5] = fma_t(cq[ch][2], pi, acc[ch][5]);
+            acc[ch][6] = fma_t(cq[ch][3], pr, acc[ch][6]);
+            acc[ch][7] = fma_t(cq[ch][3], pi, acc[ch][7]);
+        }
+        if ((c & 15) == 15) flush_stage(acc, tot);
+    }
+#pragma unroll
+    for (int ch = 0; ch < 2; ++ch) {
+        if (!(ch ? valid_b : valid_a)) continue;
+        const size_t o = (size_t)(ch ? fb : fa) * a.nbl + bl;
+        T s8[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) s8[e] = two_level_sum(acc[ch][e], tot[ch][e]);
+        // XX, XY = A + i B, YX = A - i B, YY.
+        const T v[8] = {s8[0], s8[1], s8[2] - s8[5], s8[3] + s8[4], s8[2] + s8[5], s8[3] - s8[4], s8[6], s8[7]};
+        float *v32 = reinterpret_cast<float *>(&a.vis32[o]);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v32[e] += (float)v[e];
+        if (a.vis64 != nullptr) {
+            double *v64 = reinterpret_cast<double *>(&a.vis64[o]);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) v64[e] += (double)v[e];
+        }
+    }
 }
-__global__ __launch_bounds__(256) void k_shapelet_chan32(const ShapeletChanArgs a) { shapelet_chan<false, float>(a); }
+
+// Shared beam.
+__global__ __launch_bounds__(256) void k_shapelet_chan(const ShapeletChanArgs a) { shapelet_pair<double>(a); }
+__global__ __launch_bounds__(256) void k_shapelet_chan32(const ShapeletChanArgs a) { shapelet_pair<float>(a); }
 
 // Per-tile beams: held to 128 VGPRs (4 waves per SIMD, a few spilled values) instead of the
 // 164 the compiler picks (3 waves). Measured on the 256-tile, 384-channel case: 129 -> 111 ms
@@ -1076,7 +1333,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
 
 hipError_t launch_shapelet_chan(bool per_tile, const ShapeletChanArgs &a, hipStream_t stream) {
     if (a.ncomp <= 0 || a.nbl <= 0 || a.nchan <= 0) return hipSuccess;
-    const int total = ((a.nbl + 3) / 4) * a.ngroups;
+    // Per-tile beams: a.ngroups groups of 64 channels; shared beam: groups of 128 channels.
+    const int total = ((a.nbl + 3) / 4) * (per_tile ? a.ngroups : (a.nchan + 127) / 128);
     const bool f32 = a.coh32 != nullptr;
     if (f32 && (a.sbf32 == nullptr || (per_tile && a.beam.coh_cls != nullptr && a.coh_cls32 == nullptr)))
         return hipErrorInvalidValue;

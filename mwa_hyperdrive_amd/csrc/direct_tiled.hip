@@ -25,12 +25,13 @@ constexpr int kTG = 16;   // tiles per group
 constexpr int kWin = 32;  // shapelet table window per wave and axis, cells
 constexpr int kRows = 16; // shapelet table rows staged per wave
 constexpr int kCT = 32;   // components per LDS stage
-// Channels per lane in k_direct_tiled. Fewer channels hold fewer accumulators (more waves per
-// SIMD) but share each component's phasor products, and a Gaussian's three exps, over fewer
-// channels. Measured on C2 (768 channels): points 35.1 ms with 8, 33.6 ms with 4; Gaussians
-// 19.4 ms with 8 (189 VGPRs, 2 waves per SIMD), 15.8 ms with 6 (157 VGPRs, 3 waves), 21.7 ms
-// with 4.
-constexpr int kTiledNchPoint = 4;
+// Channels per lane in k_direct_tiled (and k_direct_rect for points). Fewer channels hold fewer
+// accumulators (more waves per SIMD) but share each component's scalar coherency loads, phasor
+// reads and products, and a Gaussian's three exps, over fewer channels. Measured per C2 launch
+// (768 channels) with the Hermitian coherency table: points 21.4 ms with 4 channels, 19.4 ms
+// with 6, 17.4 ms with 8 (167 VGPRs, 3 waves per SIMD); Gaussians 15.1 ms with 4, 12.6 ms with 6
+// (161 VGPRs), 12.5 ms with 8. The FP32 Gaussians keep 6 (see launch_direct_tiled).
+constexpr int kTiledNchPoint = 8;
 constexpr int kTiledNchGaussian = 6;
 static_assert(kTiledNchPoint <= SM_NCH && kTiledNchGaussian <= SM_NCH,
               "the padded channel count covers whole chunks of SM_NCH channels");
@@ -1232,7 +1233,7 @@ __device__ __forceinline__ void shapelet_pair(const ShapeletChanArgs &a) {
                 const PackedCoeff *cp = packed + pr4.x + (pt ? pr4.y : 0);
                 const int cnt = pt ? pr4.z : pr4.y;
                 T acc_a = T(0), acc_b = T(0);
-#pragma unroll 2
+#pragma unroll 4
                 for (int k = 0; k < cnt; ++k) {
                     double val;
                     int n1, n2;

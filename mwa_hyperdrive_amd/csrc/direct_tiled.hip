@@ -24,6 +24,7 @@ namespace sm {
 constexpr int kTG = 16;   // tiles per group
 constexpr int kWin = 32;  // shapelet table window per wave and axis, cells
 constexpr int kRows = 16; // shapelet table rows staged per wave
+constexpr int kWinPair = 24;  // shapelet table window per wave and axis of the two-channel kernel, cells
 constexpr int kCT = 32;   // components per LDS stage
 // Channels per lane in k_direct_tiled (and k_direct_rect for points). Fewer channels hold fewer
 // accumulators (more waves per SIMD) but share each component's scalar coherency loads, phasor
@@ -1057,15 +1058,23 @@ __device__ __forceinline__ void shapelet_chan(const ShapeletChanArgs &a) {
 // Shared beam: one wave per baseline and 128 channels, two adjacent channels per lane
 // (channels 2 l and 2 l + 1 of the group on lane l). The table cells are read for both channels
 // at once: adjacent channels' positions differ by |k| df, below one cell (checked per component),
-// so both fall in cells i0 and i0 + 1 with i0 the lower of their two cells, and the lane reads
-// the three values T[i0], T[i0 + 1], T[i0 + 2] per axis and coefficient instead of two values
-// per channel (three quarters of the LDS bytes). A channel in cell i0 interpolates
+// so both fall in cells i0 and i0 + 1 with i0 the lower of their two cells. The staged window
+// holds each cell's value and its difference to the next cell, formed once per component, and
+// the lane reads T[i0], T[i0 + 1] - T[i0] and T[i0 + 2] - T[i0 + 1] per axis and coefficient
+// (24 bytes for two channels, against 16 per channel reading two values). A channel in cell
+// i0 interpolates
 //   u = T[i0] + (T[i0+1] - T[i0]) w,
 // as the single-channel form does, and one in cell i0 + 1 takes
 //   u = (T[i0] + (T[i0+1] - T[i0])) + (T[i0+2] - T[i0+1]) w,
 // equal to T[i0+1] + (T[i0+2] - T[i0+1]) w up to the rounding of T[i0] + (T[i0+1] - T[i0]).
 // The coherency table is in the Hermitian form (Re XX, Re XY, Im XY, Re YY; J S J^H with one
 // beam on both sides), so each channel accumulates with 8 FMAs per component.
+
+// A staged basis-table cell: the value T[i] and the difference T[i + 1] - T[i].
+template <class T> struct __attribute__((aligned(2 * sizeof(T)))) CellTD {
+    T t, d;
+};
+
 template <class T>
 __device__ __forceinline__ void shapelet_pair(const ShapeletChanArgs &a) {
     const int ngroups = (a.nchan + 127) / 128;
@@ -1088,8 +1097,9 @@ __device__ __forceinline__ void shapelet_pair(const ShapeletChanArgs &a) {
     const PackedCoeff *__restrict__ packed = a.packed;
     const T *__restrict__ sbf = pick<T>(a.sbf, a.sbf32);
     const T *__restrict__ coh_t = pick<T>(a.coh, a.coh32);
-    // Per-wave windows of the basis table (kRows rows x kWin cells per axis).
-    __shared__ T wx_lds[4][kRows][kWin], wy_lds[4][kRows][kWin];
+    // Per-wave windows of the basis table (kRows rows x kWinPair cells per axis), each cell
+    // holding T[i] and T[i + 1] - T[i]: 48 KB per block in FP64, so three blocks share a CU.
+    __shared__ CellTD<T> wx_lds[4][kRows][kWinPair], wy_lds[4][kRows][kWinPair];
     const int wv = threadIdx.x >> 6;
     // The wave's lowest and highest channel frequency and its largest step between a lane's
     // two channels (channels past the last repeat it).
@@ -1099,6 +1109,12 @@ __device__ __forceinline__ void shapelet_pair(const ShapeletChanArgs &a) {
         fmax_w = fmax(fmax_w, __shfl_xor(fmax_w, o));
         dfmax_w = fmax(dfmax_w, __shfl_xor(dfmax_w, o));
     }
+    // FP64 with the same step between the two channels on every lane (evenly spaced channels):
+    // channel b's phasor is channel a's turned by exp(i theta df / c), the turn formed once per
+    // component in the batch below, instead of a second phase reduction and sincos.
+    const double df_ref = __shfl(freq_b - freq_a, 0);
+    const bool rot = std::is_same<T, double>::value && __all(!valid_b || freq_b - freq_a == df_ref);
+    const double dfc_rot = df_ref / SM_VEL_C;
     T acc[2][8], tot[2][8];
 #pragma unroll
     for (int ch = 0; ch < 2; ++ch)
@@ -1121,7 +1137,7 @@ __device__ __forceinline__ void shapelet_pair(const ShapeletChanArgs &a) {
     const double4 *__restrict__ hd4 = reinterpret_cast<const double4 *>(a.hadec);
     // As in shapelet_chan: the wave-uniform per-component values, formed by lane l for
     // component 64 q + l and read back by lane.
-    double th_lane = 0.0, xk_lane = 0.0, yk_lane = 0.0;
+    double th_lane = 0.0, xk_lane = 0.0, yk_lane = 0.0, rs_lane = 0.0, rc_lane = 1.0;
     int wx_lane = 0, wy_lane = 0;
     auto load_batch = [&](int q) {
         const int cl = min((q << 6) + lane, a.ncomp - 1);
@@ -1130,6 +1146,7 @@ __device__ __forceinline__ void shapelet_pair(const ShapeletChanArgs &a) {
         const double2 suv =
             uv_here ? shapelet_uv(hd4[cl], x1, y1, z1, x2, y2, z2) : a.bl_uv[(size_t)cl * a.nbl + blu];
         th_lane = uvw.u * l.x + uvw.v * l.y + uvw.w * l.z;
+        if (rot) sincos_phase(th_lane * dfc_rot, rs_lane, rc_lane);
         const double xk = (suv.x * sh.x + suv.y * sh.y) * sh.z;
         const double yk = (suv.x * sh.y - suv.y * sh.x) * sh.w;
         const double xa = xk * fmin_w + SM_SBF_C, xb = xk * fmax_w + SM_SBF_C;
@@ -1154,7 +1171,13 @@ __device__ __forceinline__ void shapelet_pair(const ShapeletChanArgs &a) {
         const int wxp = __builtin_amdgcn_readlane(wx_lane, src), wyp = __builtin_amdgcn_readlane(wy_lane, src);
         T zr[2], zi[2];
         sincos_phase(theta * fc_a, zi[0], zr[0]);
-        sincos_phase(theta * fc_b, zi[1], zr[1]);
+        if (rot) {
+            const double rs = readlane_f64(rs_lane, src), rc = readlane_f64(rc_lane, src);
+            zr[1] = (T)((double)zr[0] * rc - (double)zi[0] * rs);
+            zi[1] = (T)((double)zr[0] * rs + (double)zi[0] * rc);
+        } else {
+            sincos_phase(theta * fc_b, zi[1], zr[1]);
+        }
         const double xpa = xk * freq_a + SM_SBF_C, xpb = xk * freq_b + SM_SBF_C;
         const double ypa = yk * freq_a + SM_SBF_C, ypb = yk * freq_b + SM_SBF_C;
         const double fxa = floor(xpa), fxb = floor(xpb), fya = floor(ypa), fyb = floor(ypb);
@@ -1180,13 +1203,16 @@ __device__ __forceinline__ void shapelet_pair(const ShapeletChanArgs &a) {
         // two channels of a lane are less than half a cell apart on both axes (so their cells
         // differ by at most one, with a wide margin for rounding).
         const bool near = fabs(xk) * dfmax_w < 0.5 && fabs(yk) * dfmax_w < 0.5;
-        if (near && xw + 1 <= kWin && yw + 1 <= kWin && pr4.w <= kRows) {
+        if (near && xw + 1 <= kWinPair && yw + 1 <= kWinPair && pr4.w <= kRows) {
             const int bx0 = wxp & 0xffff, by0 = wyp & 0xffff;
             const int xs = xw + 1, ys = yw + 1;  // staged cells per axis
             // As in shapelet_chan: lane -> (row lane >> lw, cell lane & (2^lw - 1)). A lane past
             // the window or past row pr4.w - 1 reads an in-bounds cell that is never written; the
-            // extra cell is at most SM_SBF_L, in the row or at the next row's first cell, and it
-            // is only multiplied by a zero weight when outside the row.
+            // last cell is at most SM_SBF_L, in the row or at the next row's first cell, and the
+            // difference it enters is only multiplied by a zero weight when outside the row.
+            // T[i + 1] comes from the next lane of the row; the difference formed with the next
+            // row's lane at cell 2^lw - 1 >= xs - 1 is never read (the loop reads differences
+            // at cells up to xs - 2).
             const int wm = max(xs, ys);
             const int lw = wm <= 4 ? 2 : (wm <= 8 ? 3 : (wm <= 16 ? 4 : 5));
             const int rstep = 64 >> lw;
@@ -1211,9 +1237,12 @@ __device__ __forceinline__ void shapelet_pair(const ShapeletChanArgs &a) {
 #pragma unroll
                 for (int i = 0; i < kBatch; ++i) {
                     const int base = i0 + i * rstep, r = base + rl;
-                    if (base < pr4.w && r < pr4.w) {
-                        if (j < xs) wx_lds[wv][r][j] = tx[i];
-                        if (j < ys) wy_lds[wv][r][j] = ty[i];
+                    if (base < pr4.w) {  // wave-uniform: every lane takes part in the shuffles
+                        const T tx1 = __shfl_down(tx[i], 1), ty1 = __shfl_down(ty[i], 1);
+                        if (r < pr4.w) {
+                            if (j < xs) wx_lds[wv][r][j] = CellTD<T>{tx[i], tx1 - tx[i]};
+                            if (j < ys) wy_lds[wv][r][j] = CellTD<T>{ty[i], ty1 - ty[i]};
+                        }
                     }
                 }
             }
@@ -1238,13 +1267,12 @@ __device__ __forceinline__ void shapelet_pair(const ShapeletChanArgs &a) {
                     double val;
                     int n1, n2;
                     load_packed_uniform(cp + k, val, n1, n2);
-                    const T x0 = wx_lds[wv][n1][jx], x1v = wx_lds[wv][n1][jx + 1], x2v = wx_lds[wv][n1][jx + 2];
-                    const T y0 = wy_lds[wv][n2][jy], y1v = wy_lds[wv][n2][jy + 1], y2v = wy_lds[wv][n2][jy + 2];
-                    const T dx01 = x1v - x0, dx12 = x2v - x1v, dy01 = y1v - y0, dy12 = y2v - y1v;
-                    const T u_a = fma_t(dx12, bx_a, fma_t(dx01, ax_a, x0));
-                    const T u_b = fma_t(dx12, bx_b, fma_t(dx01, ax_b, x0));
-                    const T v_a = fma_t(dy12, by_a, fma_t(dy01, ay_a, y0));
-                    const T v_b = fma_t(dy12, by_b, fma_t(dy01, ay_b, y0));
+                    const CellTD<T> cx = wx_lds[wv][n1][jx], cy = wy_lds[wv][n2][jy];
+                    const T dx12 = wx_lds[wv][n1][jx + 1].d, dy12 = wy_lds[wv][n2][jy + 1].d;
+                    const T u_a = fma_t(dx12, bx_a, fma_t(cx.d, ax_a, cx.t));
+                    const T u_b = fma_t(dx12, bx_b, fma_t(cx.d, ax_b, cx.t));
+                    const T v_a = fma_t(dy12, by_a, fma_t(cy.d, ay_a, cy.t));
+                    const T v_b = fma_t(dy12, by_b, fma_t(cy.d, ay_b, cy.t));
                     acc_a += ((T)val * u_a) * v_a;
                     acc_b += ((T)val * u_b) * v_b;
                 }

@@ -123,6 +123,40 @@ def test_mixed_sky_vs_oracle(beam, dead, variant):
         assert m.info()["num_unique_tiles"] > 1
 
 
+@pytest.mark.parametrize("case", ["high_order", "wide_window", "far_channels"])
+def test_shapelet_staging_exits_vs_oracle(case, variant):
+    """Shapelets that leave the shared-beam shapelet kernel's staged table window, mixed with
+    shapelets that stay in it: basis orders above the 16 staged rows (high_order), windows
+    wider than the 24 staged cells with adjacent channels under half a cell apart
+    (wide_window: 140 channels of 40 kHz, majors of 5-20 arcmin), and adjacent channels
+    many cells apart with positions clamped at the table's edge (far_channels: 1.28 MHz
+    channels, 30 arcmin majors)."""
+    obs = _mixed_obs("analytic", comps=60)
+    rng = np.random.default_rng(11)
+    arcmin = math.radians(1.0 / 60.0)
+    if case == "high_order":
+        maj = rng.uniform(0.5, 5.0, 8) * arcmin
+        orders = [(18, 3), (2, 17), (20, 20), (0, 16)]
+    elif case == "wide_window":
+        obs.freqs = 167e6 + 40e3 * np.arange(140, dtype=np.float64)
+        maj = np.linspace(5.0, 20.0, 8) * arcmin
+        orders = [(0, 0), (3, 1), (6, 4), (10, 10)]
+    else:
+        obs.freqs = 150e6 + 1.28e6 * np.arange(70, dtype=np.float64)
+        maj = np.full(8, 30.0) * arcmin
+        orders = [(0, 0), (1, 2), (5, 5), (8, 0)]
+    names = list(obs.srclist.keys())
+    for i, name in enumerate(names[:8]):
+        comp = obs.srclist[name].components[0]
+        coeffs = [(n1, n2, float(rng.normal())) for n1, n2 in orders] + [(1, 1, 0.5)]
+        comp.comp_type = Shapelet(float(maj[i]), float(maj[i] * 0.6), float(rng.uniform(0, np.pi)), coeffs)
+    v32, v64, _, _ = gpu_model(obs, 0)
+    o32, o64 = oracle_model(obs, 0)
+    err = rel_err(v64, o64)
+    print(f"case={case} variant={variant} rel err {err:.3e}")
+    assert err <= TOL_F64
+
+
 def test_uneven_channels_vs_oracle(variant):
     """Channels that are not evenly spaced take the per-channel sincos/exp kernel."""
     freqs = 167e6 + 40e3 * np.array([0, 1, 2, 4, 5, 9, 10, 11, 12, 13, 20, 21], dtype=np.float64)

@@ -345,6 +345,30 @@ __device__ __forceinline__ void acc_abh(const T *a, const T *b, T zr, T zi, T *a
         }
 }
 
+// 2x2 complex acc += A B^H.
+template <class T> __device__ __forceinline__ void acc_ab(const T *a, const T *b, T *acc) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int e = 2 * (2 * i + j);
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const T ar = a[4 * i + 2 * k], ai = a[4 * i + 2 * k + 1];
+                const T br = b[4 * j + 2 * k], bi = b[4 * j + 2 * k + 1];
+                acc[e] = fma_t(ar, br, acc[e]);
+                acc[e] = fma_t(ai, bi, acc[e]);
+                acc[e + 1] = fma_t(ai, br, acc[e + 1]);
+                acc[e + 1] = fma_t(-ar, bi, acc[e + 1]);
+            }
+        }
+}
+
+// Points: the baseline phasor factorises over the two tiles, z = K_p conj(K_q), so the staged
+// factors carry it: A = K_p J_p S (row slots) and B = K_q J_q (column slots) per channel, with
+// K_t = exp(i theta_t f / c) formed in FP64 per (component, tile slot, channel), and each lane
+// accumulates A B^H (32 FMAs per channel and component instead of 48). Gaussians keep z
+// outside: their envelope depends on the baseline, not on the tiles.
 template <int KIND, class T>
 __global__ __launch_bounds__(256) void k_pair_jones(const PairJonesArgs a, int ngroups) {
     using V4 = typename Vec4<T>::type;
@@ -402,7 +426,7 @@ __global__ __launch_bounds__(256) void k_pair_jones(const PairJonesArgs a, int n
 
     for (int c0 = c_begin; c0 < c_end; c0 += kPJC) {
         const int nc = c_end - c0 < kPJC ? c_end - c0 : kPJC;
-        {
+        if (KIND != KIND_POINT) {
             // One phasor pair per (component, tile slot): 8 x 32 = 256 threads.
             const int cc = tid >> 5, slot = tid & 31;
             double ps = 0.0, pc = 1.0, qs = 0.0, qc = 1.0;
@@ -434,6 +458,12 @@ __global__ __launch_bounds__(256) void k_pair_jones(const PairJonesArgs a, int n
             load_jones(&a.beam.jones[((size_t)s_row[slot] * a.beam.n_ufreq + a.beam.freq_map[f]) * a.beam.n_dir +
                                      a.beam.dir_offset + c],
                        j);
+            double ks = 0.0, kc = 1.0;  // the tile phasor K_t at this channel (points)
+            if (KIND == KIND_POINT) {
+                const double4 l = a.lmn[c];
+                const double th = s_uvw[slot][0] * l.x + s_uvw[slot][1] * l.y + s_uvw[slot][2] * l.z;
+                fast_sincos(th * (a.freqs[f] / SM_VEL_C), ks, kc);
+            }
             if (slot < 16) {
                 double sv[8];
                 load_jones(reinterpret_cast<const JonesF64 *>(a.coh + ((size_t)c * a.nchan_pad + f) * 8), sv);
@@ -450,15 +480,31 @@ __global__ __launch_bounds__(256) void k_pair_jones(const PairJonesArgs a, int n
                             re += jr * sr - ji * si;
                             im += jr * si + ji * sr;
                         }
-                        o[4 * i + 2 * l] = (T)re;
-                        o[4 * i + 2 * l + 1] = (T)im;
+                        if (KIND == KIND_POINT) {
+                            o[4 * i + 2 * l] = (T)(re * kc - im * ks);
+                            o[4 * i + 2 * l + 1] = (T)(re * ks + im * kc);
+                        } else {
+                            o[4 * i + 2 * l] = (T)re;
+                            o[4 * i + 2 * l + 1] = (T)im;
+                        }
                     }
+            } else if (KIND == KIND_POINT) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    o[2 * e] = (T)(j[2 * e] * kc - j[2 * e + 1] * ks);
+                    o[2 * e + 1] = (T)(j[2 * e] * ks + j[2 * e + 1] * kc);
+                }
             } else {
 #pragma unroll
                 for (int e = 0; e < 8; ++e) o[e] = (T)j[e];
             }
         }
         __syncthreads();
+        if (KIND == KIND_POINT) {
+            for (int cc = 0; cc < nc; ++cc)
+#pragma unroll
+                for (int k = 0; k < kPJN; ++k) acc_ab<T>(s_m[cc][k][r], s_m[cc][k][16 + cq], acc[k]);
+        } else
         for (int cc = 0; cc < nc; ++cc) {
             const V4 A = s_ph[cc][r];
             const V4 B = s_ph[cc][16 + cq];

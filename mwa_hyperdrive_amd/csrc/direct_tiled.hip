@@ -26,7 +26,7 @@ constexpr int kWin = 32;  // shapelet table window per wave and axis, cells
 constexpr int kRows = 16; // shapelet table rows staged per wave
 constexpr int kWinPair = 24;  // shapelet table window per wave and axis of the two-channel kernel, cells
 constexpr int kCT = 32;   // components per LDS stage
-// Channels per lane in k_direct_tiled (and k_direct_rect for points). Fewer channels hold fewer
+// Channels per lane in k_direct_tiled. Fewer channels hold fewer
 // accumulators (more waves per SIMD) but share each component's scalar coherency loads, phasor
 // reads and products, and a Gaussian's three exps, over fewer channels. Measured per C2 launch
 // (768 channels) with the Hermitian coherency table: points 21.4 ms with 4 channels, 19.4 ms
@@ -657,6 +657,10 @@ constexpr int kRT = 4 + 64;  // tile slots per block: 4 rows, then 64 columns
 // FP64: 48.1 ms with 4, 42.4 ms with 6 (151 VGPRs, 3 waves per SIMD); FP32: 28.3 ms with 4,
 // 26.8 ms with 6.
 constexpr int kRectNchGaussian = 6;
+// Points: the full-form J_r S J_0^H of a row is 64 bytes per channel of scalar loads, so wider
+// chunks cost more scalar registers and waits than they share. Measured per C5 FP64 timestep:
+// 138.1 ms with 4 channels per lane, 137.0 ms with 6, 149.9 ms with 8.
+constexpr int kRectNchPoint = 4;
 
 template <int KIND, int NCH, class T>
 __global__ __launch_bounds__(256) void k_direct_rect(const RectArgs a) {
@@ -814,20 +818,20 @@ __global__ __launch_bounds__(256) void k_direct_rect(const RectArgs a) {
 
 hipError_t launch_direct_rect(Kind kind, const RectArgs &a, hipStream_t stream) {
     if (a.ncomp <= 0 || a.n_row <= 0 || a.n_col <= 0 || a.nchan <= 0) return hipSuccess;
-    const int nch = kind == KIND_GAUSSIAN ? kRectNchGaussian : kTiledNchPoint;
+    const int nch = kind == KIND_GAUSSIAN ? kRectNchGaussian : kRectNchPoint;
     const long total = (long)((a.n_row + 3) / 4) * ((a.n_col + 63) / 64) * chunk_count(a.chunks, nch, a.nchan);
     if (a.coh_cls32 != nullptr) {
         if (kind == KIND_GAUSSIAN)
             hipLaunchKernelGGL((k_direct_rect<KIND_GAUSSIAN, kRectNchGaussian, float>), dim3((unsigned)total),
                                dim3(256), 0, stream, a);
         else
-            hipLaunchKernelGGL((k_direct_rect<KIND_POINT, kTiledNchPoint, float>), dim3((unsigned)total), dim3(256), 0,
+            hipLaunchKernelGGL((k_direct_rect<KIND_POINT, kRectNchPoint, float>), dim3((unsigned)total), dim3(256), 0,
                                stream, a);
     } else if (kind == KIND_GAUSSIAN) {
         hipLaunchKernelGGL((k_direct_rect<KIND_GAUSSIAN, kRectNchGaussian, double>), dim3((unsigned)total),
                            dim3(256), 0, stream, a);
     } else {
-        hipLaunchKernelGGL((k_direct_rect<KIND_POINT, kTiledNchPoint, double>), dim3((unsigned)total), dim3(256), 0,
+        hipLaunchKernelGGL((k_direct_rect<KIND_POINT, kRectNchPoint, double>), dim3((unsigned)total), dim3(256), 0,
                            stream, a);
     }
     return hipGetLastError();

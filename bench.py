@@ -142,7 +142,7 @@ FLOP_PER_COMPVIS_GAUSSIAN = FLOP_PER_COMPVIS_POINT + 21.0  # (kx, ky) 6, quadrat
 KERNEL_NAMES = {
     "points": "k_direct_tiled<point> (FP64 tile-pair phasor kernel)",
     "gaussians": "k_direct_tiled<gaussian> (FP64 tile-pair phasor kernel)",
-    "shapelets": "k_shapelet_chan (FP64 channel-per-lane shapelet kernel)",
+    "shapelets": "k_shapelet_chan (FP64 shapelet kernel, two channels per lane)",
 }
 
 

@@ -180,7 +180,8 @@ const char *sm_set_timing(SmModeller *m, int32_t enabled);
 const char *sm_synchronize(void);
 /* Kernel choice for the direct path: 0 (default) picks the fastest kernel for the case
  * (tile-pair phasor kernel for points and Gaussians with evenly spaced channels and one
- * shared beam, channel-per-lane kernel for shapelets); 1 forces the baseline-per-lane
+ * shared beam, channel-per-lane kernels for shapelets, two channels per lane with a shared
+ * beam); 1 forces the baseline-per-lane
  * kernels everywhere. Process-wide; used to compare the two. */
 const char *sm_set_direct_variant(int32_t variant);
 

@@ -33,11 +33,15 @@ struct TiledArgs {
 };
 
 struct ShapeletChanArgs {
+    // ngroups: groups of 64 channels of the per-tile kernels (the shared-beam kernel takes
+    // groups of 128, two channels per lane, and forms its own count).
     int nbl, nchan, nchan_pad, ncomp, ngroups;
     const double *freqs;
     const UVW *uvw;           // [nbl]
     const double4 *lmn;
     const double4 *shape;
+    // Shared beam: the Hermitian form of J S J^H, [ncomp][nchan_pad][4] (Re XX, Re XY, Im XY,
+    // Re YY). Per-tile beams: S, [ncomp][nchan_pad][8].
     const double *coh;
     const double2 *bl_uv;     // [ncomp][nbl]
     // With hadec set, the UVs are formed in the kernel from hadec [ncomp][4] (sin HA, cos HA,

@@ -274,8 +274,8 @@ __global__ __launch_bounds__(256) void k_direct(const DirectArgs a, int gx, int 
         const double theta = uvw.u * lmn.x + uvw.v * lmn.y + uvw.w * lmn.z;
         double zr = 1.0, zi = 0.0, rr = 1.0, ri = 0.0;
         if (EVEN) {
-            fast_sincos(theta * f0c, zi, zr);
-            fast_sincos(theta * dfc, ri, rr);
+            table_sincos(theta * f0c, kSinCos32, zi, zr);
+            table_sincos(theta * dfc, kSinCos32, ri, rr);
         }
         double q = 0.0, env = 1.0, er = 1.0, eg = 1.0;
         double xk = 0.0, yk = 0.0;
@@ -303,7 +303,7 @@ __global__ __launch_bounds__(256) void k_direct(const DirectArgs a, int gx, int 
                 pr = zr;
                 pi = zi;
             } else {
-                fast_sincos(theta * (freqs[f0 + k] / SM_VEL_C), pi, pr);
+                table_sincos(theta * (freqs[f0 + k] / SM_VEL_C), kSinCos32, pi, pr);
             }
             if (KIND == KIND_GAUSSIAN) {
                 const double e = EVEN ? env : exp(q * (freqs[f0 + k] * freqs[f0 + k]));

@@ -184,8 +184,8 @@ __global__ __launch_bounds__(256) void k_direct_tiled(const TiledArgs a) {
             if (cc < nc) {
                 const double4 l = lmn4[c0 + cc];
                 const double th = s_uvw[slot][0] * l.x + s_uvw[slot][1] * l.y + s_uvw[slot][2] * l.z;
-                fast_sincos(th * f0c, ps, pc);
-                fast_sincos(th * dfc, qs, qc);
+                table_sincos(th * f0c, kSinCos32, ps, pc);
+                table_sincos(th * dfc, kSinCos32, qs, qc);
             }
             V4 ph;
             ph.x = (T)pc;
@@ -433,8 +433,8 @@ __global__ __launch_bounds__(256) void k_pair_jones(const PairJonesArgs a, int n
             if (cc < nc) {
                 const double4 l = a.lmn[c0 + cc];
                 const double th = s_uvw[slot][0] * l.x + s_uvw[slot][1] * l.y + s_uvw[slot][2] * l.z;
-                fast_sincos(th * f0c, ps, pc);
-                fast_sincos(th * dfc, qs, qc);
+                table_sincos(th * f0c, kSinCos32, ps, pc);
+                table_sincos(th * dfc, kSinCos32, qs, qc);
             }
             V4 ph;
             ph.x = (T)pc;
@@ -462,7 +462,7 @@ __global__ __launch_bounds__(256) void k_pair_jones(const PairJonesArgs a, int n
             if (KIND == KIND_POINT) {
                 const double4 l = a.lmn[c];
                 const double th = s_uvw[slot][0] * l.x + s_uvw[slot][1] * l.y + s_uvw[slot][2] * l.z;
-                fast_sincos(th * (a.freqs[f] / SM_VEL_C), ks, kc);
+                table_sincos(th * (a.freqs[f] / SM_VEL_C), kSinCos32, ks, kc);
             }
             if (slot < 16) {
                 double sv[8];
@@ -720,8 +720,8 @@ __global__ __launch_bounds__(256) void k_direct_rect(const RectArgs a) {
             if (cc < nc) {
                 const double4 l = lmn4[c0 + cc];
                 const double th = s_uvw[slot][0] * l.x + s_uvw[slot][1] * l.y + s_uvw[slot][2] * l.z;
-                fast_sincos(th * f0c, ps, pc);
-                fast_sincos(th * dfc, qs, qc);
+                table_sincos(th * f0c, kSinCos32, ps, pc);
+                table_sincos(th * dfc, kSinCos32, qs, qc);
             }
             V4 ph;
             ph.x = (T)pc;

@@ -133,11 +133,11 @@ __global__ __launch_bounds__(256) void k_factored_phasors(const FactoredArgs a) 
     };
     for (int f0 = 0; f0 < a.nchan; f0 += SM_NCH) {
         double s, co;
-        fast_sincos(th * (a.freqs[f0] / SM_VEL_C), s, co);
+        table_sincos(th * (a.freqs[f0] / SM_VEL_C), kSinCos32, s, co);
         put(f0, co, s);
         if (a.even) {
             double ds, dc;
-            fast_sincos(th * ((a.freqs[f0 + 1] - a.freqs[f0]) / SM_VEL_C), ds, dc);
+            table_sincos(th * ((a.freqs[f0 + 1] - a.freqs[f0]) / SM_VEL_C), kSinCos32, ds, dc);
             for (int k = 1; k < SM_NCH && f0 + k < a.nchan; ++k) {
                 const double ns = s * dc + co * ds;
                 co = co * dc - s * ds;
@@ -146,7 +146,7 @@ __global__ __launch_bounds__(256) void k_factored_phasors(const FactoredArgs a) 
             }
         } else {
             for (int k = 1; k < SM_NCH && f0 + k < a.nchan; ++k) {
-                fast_sincos(th * (a.freqs[f0 + k] / SM_VEL_C), s, co);
+                table_sincos(th * (a.freqs[f0 + k] / SM_VEL_C), kSinCos32, s, co);
                 put(f0 + k, co, s);
             }
         }

@@ -741,11 +741,17 @@ hipError_t launch_analytic_beam(int n_tiles, const unsigned *d_delays, const dou
                                 const double *d_freqs, int n_dirs, const double *d_az, const double *d_za,
                                 JonesF64 *d_out, hipStream_t stream) {
     if (n_tiles <= 0 || n_freqs <= 0 || n_dirs <= 0) return hipSuccess;
-    const long ny = (long)n_tiles * ((n_freqs + kBeamFB - 1) / kBeamFB);
-    if (ny > 65535) return hipErrorInvalidValue;
-    dim3 grid((n_dirs + 255) / 256, (unsigned)ny);
-    hipLaunchKernelGGL(k_analytic_beam, grid, dim3(256), 0, stream, n_freqs, d_freqs, n_dirs, d_az, d_za, d_delays,
-                       d_gains, d_out);
+    // Tiles in batches that keep the grid's y extent (tiles x frequency blocks) within 65535.
+    const int nfb = (n_freqs + kBeamFB - 1) / kBeamFB;
+    if (nfb > 65535) return hipErrorInvalidValue;
+    const int per = 65535 / nfb;
+    for (int t0 = 0; t0 < n_tiles; t0 += per) {
+        const int nt = n_tiles - t0 < per ? n_tiles - t0 : per;
+        dim3 grid((n_dirs + 255) / 256, (unsigned)(nt * nfb));
+        hipLaunchKernelGGL(k_analytic_beam, grid, dim3(256), 0, stream, n_freqs, d_freqs, n_dirs, d_az, d_za,
+                           d_delays + 16 * (size_t)t0, d_gains + 32 * (size_t)t0,
+                           d_out + (size_t)t0 * n_freqs * n_dirs);
+    }
     return hipGetLastError();
 }
 

@@ -213,6 +213,7 @@ struct SmModeller {
     bool even = false;
     // device
     DevBuf d_freqs, d_xyz, d_bl_tiles, d_bl_rows, d_tile_rows, d_freq_map, d_ufreqs, d_delays, d_gains, d_sbf;
+    DevBuf d_udelays, d_ugains;  // the unique tiles' delays [n_utile][16] and gains [n_utile][32]
     DevBuf d_sbf32;  // the basis table in FP32 (precision path)
     DevBuf d_chunks4, d_chunks6, d_chunks8;  // evenly spaced channel runs of at most 4, 6 and 8 channels
     int nchunks4 = 0, nchunks6 = 0, nchunks8 = 0;
@@ -430,13 +431,12 @@ const char *prepare_beam(SmModeller *m, DevType &d, double lst, double latitude,
             return e;
         m->surrogate_fallbacks += nfb;
     }
-    // One launch per unique tile keeps the grid's y extent at n_ufreq.
-    for (int u = 0; u < m->n_utile && m->surrogate == nullptr; ++u) {
-        const int t = m->unique_tile_total[u];
-        SM_TRY(launch_analytic_beam(1, m->d_delays.as<unsigned>() + 16 * t, m->d_gains.as<double>() + 32 * t,
-                                    m->n_ufreq, m->d_ufreqs.as<double>(), n, d.az.as<double>(), d.za.as<double>(),
-                                    d.beam.as<JonesF64>() + (size_t)u * m->n_ufreq * n, 0));
-    }
+    // All unique tiles in one launch (or a few, for the grid's y extent), from their contiguous
+    // delays and gains.
+    if (m->surrogate == nullptr)
+        SM_TRY(launch_analytic_beam(m->n_utile, m->d_udelays.as<unsigned>(), m->d_ugains.as<double>(), m->n_ufreq,
+                                    m->d_ufreqs.as<double>(), n, d.az.as<double>(), d.za.as<double>(),
+                                    d.beam.as<JonesF64>(), 0));
     b.jones = d.beam.as<JonesF64>();
     if (m->n_utile == 1) {
         b.uniform_row = 0;
@@ -558,6 +558,19 @@ extern "C" const char *sm_new(const SmObsParams *p, const SmComponent *comps, co
     SM_NEW_TRY(upload(m->d_ufreqs, m->ufreqs));
     SM_NEW_TRY(upload(m->d_delays, m->delays));
     SM_NEW_TRY(upload(m->d_gains, m->gains));
+    {
+        std::vector<unsigned> ud;
+        std::vector<double> ug;
+        for (int t : m->unique_tile_total) {
+            if (16 * (size_t)t + 16 > m->delays.size() || 32 * (size_t)t + 32 > m->gains.size()) break;
+            ud.insert(ud.end(), m->delays.begin() + 16 * t, m->delays.begin() + 16 * t + 16);
+            ug.insert(ug.end(), m->gains.begin() + 32 * t, m->gains.begin() + 32 * t + 32);
+        }
+        if (!ud.empty()) {
+            SM_NEW_TRY(upload(m->d_udelays, ud));
+            SM_NEW_TRY(upload(m->d_ugains, ug));
+        }
+    }
     SM_NEW_TRY(upload(m->d_sbf, shapelet_table()));
     {
         const std::vector<int2> c4 = even_chunks(m->freqs_pad.data(), m->nchan, 4);
